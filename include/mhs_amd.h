@@ -1,0 +1,101 @@
+/* C-ABI of the MI355X mollified Mindlin half-space assembler.
+ *
+ * The reference (brendanjmeade/mhs) has no FFI: its boundary is the Python
+ * public API of src/mhs/matrices.py, re-exported at src/mhs/__init__.py:45-72,
+ * whose contract is that the assembly layer below "only ever see[s] contiguous
+ * float64 arrays and plain floats" (matrices.py:3-7), and whose designed seam
+ * for a compiled replacement is assemble.fullspace_blocks and the assemble_*
+ * functions (kernels/assemble.py:18-21, 96-105).  This header IS that seam:
+ * plain pointers and sizes, device pointers for every array, no exceptions
+ * across the boundary.  Validation, shapes, the byte ceiling and out= stay in
+ * the Python layer (mhs_amd/api.py), exactly as in matrices.py.
+ *
+ * Every entry point returns 0 on success and a negative code on failure;
+ * mhs_last_error() then holds a thread-local message.  Launches are
+ * asynchronous on the given HIP stream (NULL = the default stream).
+ */
+#ifndef MHS_AMD_H
+#define MHS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MHS_ABI_VERSION 1
+
+/* Output kinds: the five public matrix forms (matrices.py:231-580).
+ *   MHS_DISP         (n_obs, 3, n_cols, 3)     disp_matrix
+ *   MHS_STRESS       (n_obs, 3, 3, n_cols, 3)  stress/total_stress/elastic_strain
+ *   MHS_EIGEN        (n_obs, 3, 3, n_cols, 3)  eigenstress_matrix
+ *   MHS_TRACTION     (n_obs, 3, n_cols, 3)     traction_matrix
+ *   MHS_INTERACTION  (n_obs, n_cols, n_a, n_b) interaction_matrix */
+enum { MHS_DISP = 0, MHS_STRESS = 1, MHS_EIGEN = 2, MHS_TRACTION = 3, MHS_INTERACTION = 4 };
+
+/* Flags: fold surface-reaching sources (fold=True), subtract the eigenstress
+ * (subtract_eigenstress=True), also write the eigenstress to out2 (parts=True). */
+enum { MHS_FOLD = 1, MHS_SUBTRACT = 2, MHS_PARTS = 4 };
+
+typedef struct mhs_launch {
+  int32_t abi_version;      /* MHS_ABI_VERSION */
+  int32_t kind;             /* MHS_DISP .. MHS_INTERACTION */
+  int32_t order;            /* nodal order 0, 1, 2 (K = 1, 3, 6 columns per source) */
+  int32_t flags;            /* MHS_FOLD | MHS_SUBTRACT | MHS_PARTS */
+  int32_t n_a, n_b;         /* interaction receiver / source component counts */
+  int64_t n_obs, n_src, n_cols;   /* n_cols = n_src * K (element-major columns) */
+  int64_t obs_lo, obs_hi;   /* observer rows [obs_lo, obs_hi) of this launch; out row 0 = obs_lo */
+  double mu, lam;           /* Material(mu, lam) */
+  const double* obs;        /* n_obs x 3 observers (z <= 0) */
+  const double* nrm;        /* n_obs x 3 unit receiver normals (traction, interaction) */
+  const double* rb;         /* n_obs x n_a x 3 receiver frame rows (interaction) */
+  const double* sb;         /* n_src x n_b x 3 source frame rows, or NULL = Cartesian (n_b = 3) */
+  const double* src;        /* n_src x mhs_src_stride() per-source records (mhs_amd/geometry.py) */
+  const double* coef;       /* mhs_ncoef() image-kernel coefficients from mhs_fill_material */
+  const double* gl_x;       /* Gauss-Legendre nodes on [0, 1] for orders 1..192, order n at n(n-1)/2 */
+  const double* gl_w;       /* matching weights */
+  double* out;              /* output rows [obs_lo, obs_hi) in the kind's layout */
+  double* out2;             /* eigenstress part (MHS_PARTS), same layout, or NULL */
+  const int64_t* strad;     /* indices of the straddling (fold) sources, n_strad of them */
+  int64_t n_strad;
+  int64_t* fold_list;       /* scratch: 2 x fold_cap int64 (receiver, source) pairs */
+  int32_t* fold_count;      /* scratch: 2 int32 */
+  int32_t* fold_err;        /* 1 int32: bit 0 grid overflow, bit 1 list overflow (caller zeroes) */
+  int64_t fold_cap;
+} mhs_launch;
+
+/* Assemble one matrix block: replaces the per-source loops of
+ * src/mhs/kernels/assemble.py:152-331 (assemble_halfspace_disp,
+ * assemble_halfspace_total_stress, assemble_eigenstress,
+ * assemble_halfspace_traction, assemble_interaction) for the observer rows
+ * [obs_lo, obs_hi).  Columns are element-major; the continuous layout's
+ * shared columns are summed afterwards by mhs_gather. */
+int mhs_assemble(const mhs_launch* launch, void* stream);
+
+/* Continuous layout scatter-add (assemble.py:69-93, 328-330), as a gather:
+ * out[..., g, ...] = sum over the element columns idx[ptr[g]:ptr[g+1]]
+ * (ascending) of scratch[..., c, ...].  With sb (n_nodes x n_b x 3) the
+ * scratch is an interaction block with Cartesian slip (n_lead, n_elem_cols,
+ * n_a, 3) and the slip is resolved on node g's own frame rows. */
+int mhs_gather(const double* scratch, double* out, int64_t n_lead, int64_t n_elem_cols,
+               int64_t n_nodes, int tail, const int64_t* ptr, const int64_t* idx,
+               const double* sb, int n_a, int n_b, void* stream);
+
+/* Elastic strain from elastic stress in place (matrices.py:572-579). */
+int mhs_strain(double* buf, int64_t n_obs, int64_t n_cols, double mu, double lam, void* stream);
+
+/* Per-material image-kernel coefficients into host memory (mhs_ncoef()
+ * doubles): the exact rational image table with nu = lam / (2 (lam + mu)) and
+ * the scale 2 (lam + mu) / (pi mu (lam + 2 mu)) folded in
+ * (src/mhs/kernels/image.py:95-104, 276-297, 453-456). */
+int mhs_fill_material(double mu, double lam, double* host_coef);
+
+int mhs_ncoef(void);
+int mhs_src_stride(void);
+int mhs_abi_version(void);
+const char* mhs_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

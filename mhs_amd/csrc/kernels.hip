@@ -1,0 +1,342 @@
+// gfx950 kernels of the mollified Mindlin half-space assembler and the
+// C-ABI they are launched through (declared in include/mhs_amd.h).
+//
+// Launch plan of one matrix call (reference: src/mhs/kernels/assemble.py):
+//   k_pair   one thread per (observer, source node): direct + image R-family
+//            (or the folded mirror in the band) + eigenstress, written in the
+//            output's own layout;
+//   k_q      one thread per (observer, source node): the image Q-family at the
+//            budget-law Gauss order, added;
+//   k_fold_* one workgroup per folded (receiver, straddling source) pair: the
+//            Boussinesq surface integral, added;
+//   k_gather the continuous layout's shared-column sums; k_strain the elastic
+//            strain readout.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "fold.hpp"
+#include "../../include/mhs_amd.h"
+
+static thread_local char g_err[512];
+
+static int fail(const char* msg) {
+  snprintf(g_err, sizeof(g_err), "%s", msg);
+  return -1;
+}
+
+static int check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    snprintf(g_err, sizeof(g_err), "%s: %s", what, hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------ pair --
+template <int P, int MODE>   // MODE 0: stress family, 1: displacement, 2: eigenstress only
+__global__ void __launch_bounds__(256) k_pair(Params pr) {
+  const int64_t nblk = (pr.n_cols + 255) / 256;
+  const int64_t o = pr.obs_lo + blockIdx.x / nblk;
+  const int64_t col = (blockIdx.x % nblk) * 256 + threadIdx.x;
+  if (col >= pr.n_cols) return;
+  pair_item<P, MODE>(pr, o, col);
+}
+
+// --------------------------------------------------------------- Q-family --
+template <int P, bool STRESS>
+__global__ void __launch_bounds__(256) k_q(Params pr) {
+  const int64_t nblk = (pr.n_cols + 255) / 256;
+  const int64_t o = pr.obs_lo + blockIdx.x / nblk;
+  const int64_t col = (blockIdx.x % nblk) * 256 + threadIdx.x;
+  if (col >= pr.n_cols) return;
+  q_item<P, STRESS>(pr, o, col);
+}
+
+// ------------------------------------------------------------------ fold --
+// Zero the fold-list counter (a fill kernel rather than a memset node, so the
+// launch sequence is safe to capture in a graph).
+__global__ void k_zero_i32(int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// Append every (observer row, straddling source) pair inside the fold band.
+__global__ void k_fold_list(Params pr, const int64_t* __restrict__ strad, int64_t n_strad,
+                            int64_t n_rows, int64_t* __restrict__ list, int* __restrict__ count,
+                            int64_t cap) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_rows * n_strad) return;
+  const int64_t o = pr.obs_lo + t / n_strad;
+  const int64_t s = strad[t % n_strad];
+  const double* src = pr.src + s * MHS_SRC_STRIDE;
+  const double ob[3] = {pr.obs[3 * o], pr.obs[3 * o + 1], pr.obs[3 * o + 2]};
+  if (!in_fold_band(src, ob, pr.flags)) return;
+  const int slot = atomicAdd(count, 1);
+  if (slot < cap) {
+    list[2 * slot] = o;
+    list[2 * slot + 1] = s;
+  }
+}
+
+template <int P, bool STRESS>
+__global__ void __launch_bounds__(256) k_fold(Params pr, const int64_t* __restrict__ list,
+                                              const int* __restrict__ count, int64_t cap,
+                                              int* __restrict__ err) {
+  constexpr int K = (P + 1) * (P + 2) / 2;
+  constexpr int NA = STRESS ? 18 : 9;
+  __shared__ FoldGrid G;
+  __shared__ double sh[3];
+  __shared__ double red[NA][8];
+  __shared__ double u6[6], w6[6], u4[4], w4[4];
+  const int tid = threadIdx.x;
+  int n_items = *count;
+  if (n_items > cap) {
+    if (blockIdx.x == 0 && tid == 0) atomicOr(err, 2);
+    n_items = (int)cap;
+  }
+  if (tid < 6) { u6[tid] = pr.gl_x[15 + tid]; w6[tid] = pr.gl_w[15 + tid]; }
+  if (tid < 4) { u4[tid] = pr.gl_x[6 + tid]; w4[tid] = pr.gl_w[6 + tid]; }
+  for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+    const int64_t o = list[2 * item], s = list[2 * item + 1];
+    const double* src = pr.src + s * MHS_SRC_STRIDE;
+    const double eps = src[S_EPS];
+    const double xh0 = pr.obs[3 * o], xh1 = pr.obs[3 * o + 1];
+    const double zeta = fmax(0.0, -pr.obs[3 * o + 2]);
+    __syncthreads();
+    if (tid == 0) {
+      fold_grid_build(G, xh0, xh1, zeta, src[S_CREASE], src[S_CREASE + 1], src[S_CREASE_R], eps,
+                      src[S_DIAM]);
+      if (G.overflow) atomicOr(err, 1);
+    }
+    __syncthreads();
+    for (int k = 0; k < K; ++k) {
+      if (tid == 0) surface_load<P>(pr, src, k, xh0, xh1, sh);
+      __syncthreads();
+      double acc[NA];
+#pragma unroll
+      for (int t = 0; t < NA; ++t) acc[t] = 0.0;
+      const double s0 = sh[0], s1 = sh[1], s2 = sh[2];
+      for (int g = tid; g < G.total; g += blockDim.x) {
+        double x, y, w;
+        fold_grid_point(G, xh0, xh1, u6, w6, u4, w4, g, x, y, w);
+        double sl[3];
+        surface_load<P>(pr, src, k, x, y, sl);
+        sl[0] -= s0; sl[1] -= s1; sl[2] -= s2;
+        const double dx = xh0 - x, dy = xh1 - y;
+        if (STRESS) {
+          double S[6];
+          point_stress(dx, dy, zeta, pr.nu, S);
+#pragma unroll
+          for (int c = 0; c < 6; ++c)
+#pragma unroll
+            for (int m = 0; m < 3; ++m) acc[c * 3 + m] = fma(w * S[c], sl[m], acc[c * 3 + m]);
+        } else {
+          double U[3];
+          point_disp(dx, dy, zeta, pr.mu, pr.nu, U);
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int m = 0; m < 3; ++m) acc[c * 3 + m] = fma(w * U[c], sl[m], acc[c * 3 + m]);
+        }
+      }
+      // block reduction: wave shuffle, then across the 4 waves
+#pragma unroll
+      for (int t = 0; t < NA; ++t) {
+        double v = acc[t];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if ((tid & 63) == 0) red[t][tid >> 6] = v;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        const int nw = (blockDim.x + 63) / 64;
+        double tot[NA];
+        for (int t = 0; t < NA; ++t) {
+          double v = 0.0;
+          for (int wv = 0; wv < nw; ++wv) v += red[t][wv];
+          tot[t] = v;
+        }
+        const int64_t col = s * K + k;
+        if (STRESS) {
+          double srr, szz;
+          disk_stress(G.R1, zeta, pr.nu, srr, szz);
+          // components (xx, yy, xy, xz, yz, zz) -> H[a][b][m]
+          double H[27];
+          const int ci[3][3] = {{0, 2, 3}, {2, 1, 4}, {3, 4, 5}};
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+              for (int m = 0; m < 3; ++m) {
+                double disk = (a == b) ? (a == 2 ? szz : srr) : 0.0;
+                H[(a * 3 + b) * 3 + m] = tot[ci[a][b] * 3 + m] + sh[m] * disk;
+              }
+          emit_stress(pr, o, col, s, H, 0.0, 0.0, src + S_N, src + S_N, true);
+        } else {
+          const double uz = disk_uz(G.R1, zeta, pr.mu, pr.nu);
+          double U[9];
+          for (int i = 0; i < 3; ++i)
+            for (int m = 0; m < 3; ++m) U[i * 3 + m] = tot[i * 3 + m] + (i == 2 ? sh[m] * uz : 0.0);
+          emit_disp(pr, o, col, U, true);
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------- gather --
+// Continuous layout: out[..., g, ...] = sum over the element columns c of
+// shared node g (CSR, ascending) of scratch[..., c, ...].  Interaction
+// additionally resolves the Cartesian slip on node g's own frame.
+__global__ void k_gather(const double* __restrict__ scr, double* __restrict__ out, int64_t n_lead,
+                         int64_t n_elem_cols, int64_t n_nodes, int tail, const int64_t* __restrict__ ptr,
+                         const int64_t* __restrict__ idx, const double* __restrict__ sb, int n_a, int n_b) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sb == nullptr) {
+    // layout (n_lead, n_cols, tail)
+    const int64_t total = n_lead * n_nodes * tail;
+    if (t >= total) return;
+    const int64_t e = t % tail;
+    const int64_t g = (t / tail) % n_nodes;
+    const int64_t L = t / (tail * n_nodes);
+    double acc = 0.0;
+    for (int64_t q = ptr[g]; q < ptr[g + 1]; ++q) acc += scr[(L * n_elem_cols + idx[q]) * tail + e];
+    out[t] = acc;
+  } else {
+    // interaction: scratch (n_obs, n_elem_cols, n_a, 3) -> out (n_obs, n_nodes, n_a, n_b)
+    const int64_t total = n_lead * n_nodes * n_a * n_b;
+    if (t >= total) return;
+    const int b = (int)(t % n_b);
+    const int a = (int)((t / n_b) % n_a);
+    const int64_t g = (t / ((int64_t)n_a * n_b)) % n_nodes;
+    const int64_t L = t / ((int64_t)n_a * n_b * n_nodes);
+    const double* bv = sb + (g * n_b + b) * 3;
+    double acc = 0.0;
+    for (int64_t q = ptr[g]; q < ptr[g + 1]; ++q) {
+      const double* v = scr + ((L * n_elem_cols + idx[q]) * n_a + a) * 3;
+      acc += v[0] * bv[0] + v[1] * bv[1] + v[2] * bv[2];
+    }
+    out[t] = acc;
+  }
+}
+
+// Elastic strain from stress (matrices.py:572-579): tr(e) = tr(sig)/(3 lam + 2 mu),
+// e = (sig - lam tr(e) I) / (2 mu), in place on (n_obs, 3, 3, n_cols, 3).
+__global__ void k_strain(double* __restrict__ buf, int64_t n_obs, int64_t n_cols, double mu, double lam) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_obs * n_cols * 3) return;
+  const int64_t j = t % 3;
+  const int64_t c = (t / 3) % n_cols;
+  const int64_t o = t / (3 * n_cols);
+  auto at = [&](int a, int b) -> double& { return buf[((o * 9 + a * 3 + b) * n_cols + c) * 3 + j]; };
+  const double tre = (at(0, 0) + at(1, 1) + at(2, 2)) / (3.0 * lam + 2.0 * mu);
+  for (int a = 0; a < 3; ++a) at(a, a) -= lam * tre;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) at(a, b) /= (2.0 * mu);
+}
+
+template <int P>
+static int launch_all(const mhs_launch* L, const Params& pr, hipStream_t st) {
+  const int64_t n_rows = L->obs_hi - L->obs_lo;
+  const int64_t nblk = (pr.n_cols + 255) / 256;
+  const int64_t blocks = n_rows * nblk;
+  if (blocks <= 0) return 0;
+  if (blocks > 0x7fffffffLL) return fail("mhs_assemble: launch too large; split the observer rows");
+  const dim3 grid((unsigned)blocks), block(256);
+  const bool stress = pr.kind != K_DISP;
+  if (pr.kind == K_DISP) hipLaunchKernelGGL((k_pair<P, 1>), grid, block, 0, st, pr);
+  else if (pr.kind == K_EIGEN) hipLaunchKernelGGL((k_pair<P, 2>), grid, block, 0, st, pr);
+  else hipLaunchKernelGGL((k_pair<P, 0>), grid, block, 0, st, pr);
+  if (check(hipGetLastError(), "k_pair launch")) return -1;
+  if (pr.kind == K_EIGEN) return 0;
+  if (stress) hipLaunchKernelGGL((k_q<P, true>), grid, block, 0, st, pr);
+  else hipLaunchKernelGGL((k_q<P, false>), grid, block, 0, st, pr);
+  if (check(hipGetLastError(), "k_q launch")) return -1;
+  if ((pr.flags & F_FOLD) && L->n_strad > 0) {
+    if (!L->fold_list || !L->fold_count || !L->fold_err) return fail("mhs_assemble: fold buffers missing");
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, L->fold_count, 2);
+    const int64_t nt = n_rows * L->n_strad;
+    hipLaunchKernelGGL(k_fold_list, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, pr, L->strad,
+                       L->n_strad, n_rows, L->fold_list, L->fold_count, L->fold_cap);
+    if (check(hipGetLastError(), "k_fold_list launch")) return -1;
+    const dim3 fg(2048);
+    if (stress) hipLaunchKernelGGL((k_fold<P, true>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, L->fold_err);
+    else hipLaunchKernelGGL((k_fold<P, false>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, L->fold_err);
+    if (check(hipGetLastError(), "k_fold launch")) return -1;
+  }
+  return 0;
+}
+
+// ----------------------------------------------------------------- C-ABI --
+extern "C" {
+
+const char* mhs_last_error(void) { return g_err; }
+
+int mhs_abi_version(void) { return MHS_ABI_VERSION; }
+
+int mhs_ncoef(void) { return MHS_NCOEF; }
+
+int mhs_src_stride(void) { return MHS_SRC_STRIDE; }
+
+int mhs_fill_material(double mu, double lam, double* host_coef) {
+  if (!(mu > 0.0) || !(lam > -2.0 * mu / 3.0)) return fail("mhs_fill_material: inadmissible (mu, lam)");
+  if (!host_coef) return fail("mhs_fill_material: null output");
+  mhs_fill_coeffs(mu, lam, host_coef);
+  return 0;
+}
+
+int mhs_assemble(const mhs_launch* L, void* stream) {
+  if (!L) return fail("mhs_assemble: null launch descriptor");
+  if (L->abi_version != MHS_ABI_VERSION) return fail("mhs_assemble: ABI version mismatch");
+  if (L->order < 0 || L->order > 2) return fail("mhs_assemble: order must be 0, 1 or 2");
+  if (L->kind < K_DISP || L->kind > K_INTERACTION) return fail("mhs_assemble: unknown kind");
+  if (!L->obs || !L->src || !L->coef || !L->gl_x || !L->gl_w || !L->out)
+    return fail("mhs_assemble: null device pointer");
+  if ((L->kind == K_TRACTION || L->kind == K_INTERACTION) && !L->nrm)
+    return fail("mhs_assemble: traction/interaction need receiver normals");
+  if (L->kind == K_INTERACTION && (!L->rb || L->n_a < 1 || L->n_a > 3 || L->n_b < 1 || L->n_b > 3))
+    return fail("mhs_assemble: interaction needs 1..3 receiver and source components");
+  if (L->kind == K_INTERACTION && !L->sb && L->n_b != 3)
+    return fail("mhs_assemble: identity source basis needs n_b = 3");
+  if ((L->flags & F_PARTS) && (L->kind != K_STRESS || !L->out2))
+    return fail("mhs_assemble: parts needs the stress kind and a second output");
+  if (L->obs_lo < 0 || L->obs_hi > L->n_obs || L->obs_lo > L->obs_hi)
+    return fail("mhs_assemble: bad observer row range");
+  const int K = (L->order + 1) * (L->order + 2) / 2;
+  if (L->n_cols != L->n_src * K) return fail("mhs_assemble: n_cols must be n_src * K");
+  Params pr;
+  pr.obs = L->obs; pr.nrm = L->nrm; pr.rb = L->rb; pr.sb = L->sb; pr.src = L->src;
+  pr.coef = L->coef; pr.gl_x = L->gl_x; pr.gl_w = L->gl_w; pr.out = L->out; pr.out2 = L->out2;
+  pr.n_obs = L->n_obs; pr.n_src = L->n_src; pr.n_cols = L->n_cols; pr.obs_lo = L->obs_lo;
+  pr.kind = L->kind; pr.flags = L->flags; pr.n_a = L->n_a; pr.n_b = L->n_b;
+  pr.mu = L->mu; pr.lam = L->lam; pr.nu = 0.5 * L->lam / (L->lam + L->mu);
+  hipStream_t st = (hipStream_t)stream;
+  switch (L->order) {
+    case 0: return launch_all<0>(L, pr, st);
+    case 1: return launch_all<1>(L, pr, st);
+    default: return launch_all<2>(L, pr, st);
+  }
+}
+
+int mhs_gather(const double* scratch, double* out, int64_t n_lead, int64_t n_elem_cols, int64_t n_nodes,
+               int tail, const int64_t* ptr, const int64_t* idx, const double* sb, int n_a, int n_b,
+               void* stream) {
+  if (!scratch || !out || !ptr || !idx) return fail("mhs_gather: null device pointer");
+  const int64_t total = sb ? n_lead * n_nodes * n_a * n_b : n_lead * n_nodes * tail;
+  if (total <= 0) return 0;
+  const int64_t blocks = (total + 255) / 256;
+  if (blocks > 0x7fffffffLL) return fail("mhs_gather: too large");
+  hipLaunchKernelGGL(k_gather, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, scratch, out,
+                     n_lead, n_elem_cols, n_nodes, tail, ptr, idx, sb, n_a, n_b);
+  return check(hipGetLastError(), "k_gather launch");
+}
+
+int mhs_strain(double* buf, int64_t n_obs, int64_t n_cols, double mu, double lam, void* stream) {
+  if (!buf) return fail("mhs_strain: null device pointer");
+  const int64_t total = n_obs * n_cols * 3;
+  if (total <= 0) return 0;
+  hipLaunchKernelGGL(k_strain, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, (hipStream_t)stream, buf,
+                     n_obs, n_cols, mu, lam);
+  return check(hipGetLastError(), "k_strain launch");
+}
+
+}  // extern "C"

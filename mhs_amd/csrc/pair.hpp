@@ -1,0 +1,366 @@
+// Per-pair evaluation: direct + image (or folded mirror) + eigenstress terms,
+// the Q-family point sum, and the output epilogues.
+//
+// Reference: src/mhs/kernels/assemble.py:96-331, image.py:170-254, fold.py:84-118.
+#pragma once
+#include "core.hpp"
+#include "gen_image.inc"
+
+enum Kind { K_DISP = 0, K_STRESS = 1, K_EIGEN = 2, K_TRACTION = 3, K_INTERACTION = 4 };
+enum Flag { F_FOLD = 1, F_SUBTRACT = 2, F_PARTS = 4 };
+
+struct Params {
+  const double* obs;     // n_obs x 3
+  const double* nrm;     // n_obs x 3 (traction, interaction)
+  const double* rb;      // n_obs x n_a x 3 (interaction)
+  const double* sb;      // n_src x n_b x 3 (interaction) or null = identity
+  const double* src;     // n_src x MHS_SRC_STRIDE
+  const double* coef;    // MHS_NCOEF image coefficients for (mu, lam)
+  const double* gl_x;    // Gauss-Legendre nodes on [0, 1], orders 1..192
+  const double* gl_w;
+  double* out;
+  double* out2;
+  int64_t n_obs, n_src, n_cols;
+  int64_t obs_lo;        // first observer row of this launch
+  int kind, flags, n_a, n_b;
+  double mu, lam, nu;
+};
+
+// Fold classification of one (receiver, source) pair (fold.py:84-118): the
+// source straddles and the receiver is shallower than 10 eps and within
+// 60 eps of the crease centre.
+MHS_INLINE bool in_fold_band(const double* __restrict__ src, const double o[3], int flags) {
+  if (!(flags & F_FOLD) || src[S_STRADDLE] == 0.0) return false;
+  const double eps = src[S_EPS];
+  const double dx = o[0] - src[S_CREASE], dy = o[1] - src[S_CREASE + 1];
+  return (-o[2] < 10.0 * eps) && (hypot(dx, dy) < 60.0 * eps);
+}
+
+MHS_INLINE void reflected_vertices(const double* __restrict__ src, double v[9]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    v[3 * i] = src[S_V + 3 * i];
+    v[3 * i + 1] = src[S_V + 3 * i + 1];
+    v[3 * i + 2] = -src[S_V + 3 * i + 2];
+  }
+}
+
+// Hooke tensor C:sym(slip (x) n) per unit slip j (assemble.py:123-128).
+MHS_INLINE double hooke(const double* n, int a, int b, int j, double mu, double lam) {
+  return (a == b ? lam * n[j] : 0.0) + mu * ((a == j ? n[b] : 0.0) + (b == j ? n[a] : 0.0));
+}
+
+// Total stress of node k at o (direct + image R-family, or the folded mirror
+// in the band) and the blob weights of the eigenstress terms.
+template <int P>
+MHS_INLINE void pair_stress(const Params& pr, const double* __restrict__ src, int k, const double o[3],
+                            bool band, bool want_H, double H[27], double& phi_d, double& phi_m) {
+  const double eps = src[S_EPS];
+  const Mat mt{pr.mu, pr.lam, pr.nu};
+  double W[6][21];
+  double X1, X2;
+  {
+    Frame f;
+    load_frame(src, false, f);
+    const double z = weighted_tables<SpecH<P>, P>(f, src + S_V, k, o, eps, pr.gl_x, pr.gl_w, W, X1, X2);
+    phi_d = direct_E(W, eps);
+    if (want_H) direct_H(f, W, z, eps, mt, H);
+  }
+  phi_m = 0.0;
+  Frame g;
+  load_frame(src, true, g);
+  double vi[9];
+  reflected_vertices(src, vi);
+  if (band) {
+    // mirror element, full space, slip columns signed (-1, -1, +1) (fold.py:370-380)
+    const double z = weighted_tables<SpecH<P>, P>(g, vi, k, o, eps, pr.gl_x, pr.gl_w, W, X1, X2);
+    phi_m = direct_E(W, eps);
+    if (want_H) {
+      double Hm[27];
+      direct_H(g, W, z, eps, mt, Hm);
+#pragma unroll
+      for (int t = 0; t < 27; ++t) H[t] += ((t % 3) == 2 ? Hm[t] : -Hm[t]);
+    }
+  } else if (want_H) {
+    // image R-family in closed form on the reflected triangle (image.py:435-497)
+    const double zp = weighted_tables<SpecI<P>, P>(g, vi, k, o, eps, pr.gl_x, pr.gl_w, W, X1, X2);
+    const double al[3] = {-g.e1[0], -g.e1[1], -g.e1[2]};
+    const double be[3] = {-g.e2[0], -g.e2[1], -g.e2[2]};
+    const double ga[3] = {zp * g.n[0], zp * g.n[1], zp * g.n[2]};
+    double DDG[81];
+#pragma unroll
+    for (int t = 0; t < 81; ++t) DDG[t] = 0.0;
+    r_ddg(W[2], W[3], W[4], W[5], al, be, ga, o[2], pr.coef, DDG);
+    const double nrm[3] = {src[S_N], src[S_N + 1], src[S_N + 2]};
+    double Hi[27];
+    contract_H(DDG, nrm, pr.mu, pr.lam, Hi);
+#pragma unroll
+    for (int t = 0; t < 27; ++t) H[t] += Hi[t];
+  }
+}
+
+template <int P>
+MHS_INLINE void pair_disp(const Params& pr, const double* __restrict__ src, int k, const double o[3],
+                          bool band, double U[9]) {
+  const double eps = src[S_EPS];
+  const Mat mt{pr.mu, pr.lam, pr.nu};
+  double W[6][21];
+  double X1, X2;
+  {
+    Frame f;
+    load_frame(src, false, f);
+    const double z = weighted_tables<SpecU<P>, P>(f, src + S_V, k, o, eps, pr.gl_x, pr.gl_w, W, X1, X2);
+    direct_U(f, W, z, eps, mt, U);
+  }
+  Frame g;
+  load_frame(src, true, g);
+  double vi[9];
+  reflected_vertices(src, vi);
+  if (band) {
+    const double z = weighted_tables<SpecU<P>, P>(g, vi, k, o, eps, pr.gl_x, pr.gl_w, W, X1, X2);
+    double Um[9];
+    direct_U(g, W, z, eps, mt, Um);
+#pragma unroll
+    for (int t = 0; t < 9; ++t) U[t] += ((t % 3) == 2 ? Um[t] : -Um[t]);
+  } else {
+    const double zp = weighted_tables<SpecI<P>, P>(g, vi, k, o, eps, pr.gl_x, pr.gl_w, W, X1, X2);
+    const double al[3] = {-g.e1[0], -g.e1[1], -g.e1[2]};
+    const double be[3] = {-g.e2[0], -g.e2[1], -g.e2[2]};
+    const double ga[3] = {zp * g.n[0], zp * g.n[1], zp * g.n[2]};
+    double DG[27];
+#pragma unroll
+    for (int t = 0; t < 27; ++t) DG[t] = 0.0;
+    r_dg(W[2], W[3], W[4], W[5], al, be, ga, o[2], pr.coef, DG);
+    const double nrm[3] = {src[S_N], src[S_N + 1], src[S_N + 2]};
+    double Ui[9];
+    contract_U(DG, nrm, pr.mu, pr.lam, Ui);
+#pragma unroll
+    for (int t = 0; t < 9; ++t) U[t] += Ui[t];
+  }
+}
+
+// Exact point-to-triangle distance (image.py:170-208).
+MHS_INLINE double triangle_distance(const double o[3], const double* v) {
+  const double a[3] = {v[0], v[1], v[2]};
+  double ab[3], ac[3], ap[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) { ab[i] = v[3 + i] - a[i]; ac[i] = v[6 + i] - a[i]; ap[i] = o[i] - a[i]; }
+  const double nx = ab[1] * ac[2] - ab[2] * ac[1];
+  const double ny = ab[2] * ac[0] - ab[0] * ac[2];
+  const double nz = ab[0] * ac[1] - ab[1] * ac[0];
+  const double d20 = ap[0] * ab[0] + ap[1] * ab[1] + ap[2] * ab[2];
+  const double d21 = ap[0] * ac[0] + ap[1] * ac[1] + ap[2] * ac[2];
+  const double d00 = ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2];
+  const double d01 = ab[0] * ac[0] + ab[1] * ac[1] + ab[2] * ac[2];
+  const double d11 = ac[0] * ac[0] + ac[1] * ac[1] + ac[2] * ac[2];
+  const double den = d00 * d11 - d01 * d01;
+  const double vv = (d11 * d20 - d01 * d21) / den;
+  const double ww = (d00 * d21 - d01 * d20) / den;
+  const double uu = 1.0 - vv - ww;
+  if (uu >= 0.0 && vv >= 0.0 && ww >= 0.0)
+    return fabs(ap[0] * nx + ap[1] * ny + ap[2] * nz) / sqrt(nx * nx + ny * ny + nz * nz);
+  double best = 1e300;
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    const double* p0 = v + 3 * e;
+    const double* p1 = v + 3 * ((e + 1) % 3);
+    double ev[3], d[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) { ev[i] = p1[i] - p0[i]; d[i] = o[i] - p0[i]; }
+    double t = (d[0] * ev[0] + d[1] * ev[1] + d[2] * ev[2]) / (ev[0] * ev[0] + ev[1] * ev[1] + ev[2] * ev[2]);
+    t = fmin(fmax(t, 0.0), 1.0);
+    const double cx = o[0] - (p0[0] + t * ev[0]);
+    const double cy = o[1] - (p0[1] + t * ev[1]);
+    const double cz = o[2] - (p0[2] + t * ev[2]);
+    best = fmin(best, sqrt(cx * cx + cy * cy + cz * cz));
+  }
+  return best;
+}
+
+// Q-family Gauss order of the pair (image.py:211-254).
+template <int P>
+MHS_INLINE int q_order(const double* __restrict__ src, const double o[3]) {
+  double vi[9];
+  reflected_vertices(src, vi);
+  const double delta = triangle_distance(o, vi);
+  const double eps = src[S_EPS];
+  const double nq = ceil(10.0 * src[S_L] / sqrt(delta * delta + eps * eps)) + P;
+  return (int)fmin(fmax(nq, 8.0), 192.0);
+}
+
+// Q-family sum for node k (image.py:315-377): accumulates DDG (stress) or DG.
+template <int P, bool STRESS>
+MHS_INLINE void q_family(const Params& pr, const double* __restrict__ src, int k, const double o[3],
+                         int nq, double* acc) {
+  const double* v = src + S_V;
+  const double eps2 = src[S_EPS] * src[S_EPS];
+  const double ar2 = 2.0 * src[S_AREA];
+  const int off = nq * (nq - 1) / 2;
+  const double* gx = pr.gl_x + off;
+  const double* gw = pr.gl_w + off;
+  for (int i = 0; i < nq; ++i) {
+    const double gi = gx[i], wi = gw[i];
+    for (int j = 0; j < nq; ++j) {
+      const double x1 = gi, x2 = gx[j] * (1.0 - gi);
+      const double w = wi * gw[j] * (1.0 - gi) * ar2;
+      const double l0 = 1.0 - x1 - x2;
+      const double y0 = l0 * v[0] + x1 * v[3] + x2 * v[6];
+      const double y1 = l0 * v[1] + x1 * v[4] + x2 * v[7];
+      const double y2 = l0 * v[2] + x1 * v[5] + x2 * v[8];
+      const double wn = w * shape_at<P>(k, l0, x1, x2);
+      const double d1 = o[0] - y0, d2 = o[1] - y1, d3 = o[2] + y2;
+      const double r2 = sqrt(d1 * d1 + d2 * d2 + d3 * d3 + eps2);
+      const double ir = 1.0 / r2;
+      const double iq = 1.0 / (r2 - d3);
+      if (STRESS) q_ddg_point(d1, d2, d3, ir, iq, wn, pr.coef, acc);
+      else q_dg_point(d1, d2, d3, ir, iq, wn, pr.coef, acc);
+    }
+  }
+}
+
+// ------------------------------------------------------------- epilogues --
+// Write (or add) the total stress H of one (observer row o, column col) into
+// the output form of pr.kind.  eig terms: phi_d with the element normal nd,
+// phi_m with the mirror normal nm and the mirror slip sign.
+MHS_INLINE void emit_stress(const Params& pr, int64_t o, int64_t col, int64_t s, const double* H,
+                            double phi_d, double phi_m, const double* nd, const double* nm,
+                            bool add) {
+  double E[27];
+  const bool eig = !add && (pr.kind == K_EIGEN || (pr.flags & (F_SUBTRACT | F_PARTS)));
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double e = 0.0;
+        if (eig) {
+          e = phi_d * hooke(nd, a, b, j, pr.mu, pr.lam);
+          if (phi_m != 0.0) e += phi_m * hooke(nm, a, b, j, pr.mu, pr.lam) * (j == 2 ? 1.0 : -1.0);
+        }
+        E[(a * 3 + b) * 3 + j] = e;
+      }
+  const int64_t oo = o - pr.obs_lo;
+  if (pr.kind == K_STRESS || pr.kind == K_EIGEN) {
+    const bool sub = (pr.flags & F_SUBTRACT) != 0;
+#pragma unroll
+    for (int ab = 0; ab < 9; ++ab)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const int64_t idx = ((oo * 9 + ab) * pr.n_cols + col) * 3 + j;
+        const double val = (pr.kind == K_EIGEN) ? E[ab * 3 + j] : (H[ab * 3 + j] - (sub ? E[ab * 3 + j] : 0.0));
+        if (add) pr.out[idx] += val; else pr.out[idx] = val;
+        if (!add && (pr.flags & F_PARTS) && pr.kind == K_STRESS) pr.out2[idx] = E[ab * 3 + j];
+      }
+    return;
+  }
+  // traction t[i][j] = sum_b (H - E)[i][b][j] n_b
+  const double* n = pr.nrm + 3 * o;
+  const bool sub = (pr.flags & F_SUBTRACT) != 0;
+  double t[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double acc = 0.0;
+#pragma unroll
+      for (int b = 0; b < 3; ++b) acc += (H[(i * 3 + b) * 3 + j] - (sub ? E[(i * 3 + b) * 3 + j] : 0.0)) * n[b];
+      t[i * 3 + j] = acc;
+    }
+  if (pr.kind == K_TRACTION) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const int64_t idx = ((oo * 3 + i) * pr.n_cols + col) * 3 + j;
+        if (add) pr.out[idx] += t[i * 3 + j]; else pr.out[idx] = t[i * 3 + j];
+      }
+    return;
+  }
+  // interaction: out[o, col, a, b] = rb[o, a, :] . t . sb[s, b, :]
+  const double* rb = pr.rb + (int64_t)3 * pr.n_a * o;
+  for (int a = 0; a < pr.n_a; ++a) {
+    double ta[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      ta[j] = rb[3 * a] * t[j] + rb[3 * a + 1] * t[3 + j] + rb[3 * a + 2] * t[6 + j];
+    for (int b = 0; b < pr.n_b; ++b) {
+      double v;
+      if (pr.sb) {
+        const double* sb = pr.sb + ((int64_t)s * pr.n_b + b) * 3;
+        v = ta[0] * sb[0] + ta[1] * sb[1] + ta[2] * sb[2];
+      } else {
+        v = ta[b];
+      }
+      const int64_t idx = ((oo * pr.n_cols + col) * pr.n_a + a) * pr.n_b + b;
+      if (add) pr.out[idx] += v; else pr.out[idx] = v;
+    }
+  }
+}
+
+MHS_INLINE void emit_disp(const Params& pr, int64_t o, int64_t col, const double* U, bool add) {
+  const int64_t oo = o - pr.obs_lo;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int64_t idx = ((oo * 3 + i) * pr.n_cols + col) * 3 + j;
+      if (add) pr.out[idx] += U[i * 3 + j]; else pr.out[idx] = U[i * 3 + j];
+    }
+}
+
+// ------------------------------------------------------------- work items --
+// One (observer row o, element column col) of the pair pass.
+template <int P, int MODE>   // MODE 0: stress family, 1: displacement, 2: eigenstress only
+MHS_INLINE void pair_item(const Params& pr, int64_t o, int64_t col) {
+  constexpr int K = (P + 1) * (P + 2) / 2;
+  const int64_t s = col / K;
+  const int k = (int)(col - s * K);
+  const double* src = pr.src + s * MHS_SRC_STRIDE;
+  const double ob[3] = {pr.obs[3 * o], pr.obs[3 * o + 1], pr.obs[3 * o + 2]};
+  const bool band = in_fold_band(src, ob, pr.flags);
+  if (MODE == 1) {
+    double U[9];
+    pair_disp<P>(pr, src, k, ob, band, U);
+    emit_disp(pr, o, col, U, false);
+  } else {
+    double H[27];
+    double phi_d, phi_m;
+    pair_stress<P>(pr, src, k, ob, band, MODE == 0, H, phi_d, phi_m);
+    if (MODE == 2) {
+#pragma unroll
+      for (int t = 0; t < 27; ++t) H[t] = 0.0;
+    }
+    emit_stress(pr, o, col, s, H, phi_d, phi_m, src + S_N, src + S_IN, false);
+  }
+}
+
+// One (observer row, element column) of the Q-family pass (added).
+template <int P, bool STRESS>
+MHS_INLINE void q_item(const Params& pr, int64_t o, int64_t col) {
+  constexpr int K = (P + 1) * (P + 2) / 2;
+  const int64_t s = col / K;
+  const int k = (int)(col - s * K);
+  const double* src = pr.src + s * MHS_SRC_STRIDE;
+  const double ob[3] = {pr.obs[3 * o], pr.obs[3 * o + 1], pr.obs[3 * o + 2]};
+  if (in_fold_band(src, ob, pr.flags)) return;       // the fold replaces R + Q there
+  const int nq = q_order<P>(src, ob);
+  const double* nrm = src + S_N;
+  if (STRESS) {
+    double DDG[81];
+#pragma unroll
+    for (int t = 0; t < 81; ++t) DDG[t] = 0.0;
+    q_family<P, true>(pr, src, k, ob, nq, DDG);
+    double H[27];
+    contract_H(DDG, nrm, pr.mu, pr.lam, H);
+    emit_stress(pr, o, col, s, H, 0.0, 0.0, nrm, nrm, true);
+  } else {
+    double DG[27];
+#pragma unroll
+    for (int t = 0; t < 27; ++t) DG[t] = 0.0;
+    q_family<P, false>(pr, src, k, ob, nq, DG);
+    double U[9];
+    contract_U(DG, nrm, pr.mu, pr.lam, U);
+    emit_disp(pr, o, col, U, true);
+  }
+}

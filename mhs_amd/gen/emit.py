@@ -1,0 +1,252 @@
+"""Emit the generated HIP device code for the image families.
+
+Reads ``mhs_amd/data/mindlin_table.json`` (derived by ``mindlin_table.py``)
+and writes ``mhs_amd/csrc/gen_image.inc`` containing
+
+* ``MHS_NCOEF`` and the host routine ``mhs_fill_coeffs(mu, lam, c)`` that
+  turns the exact rational records into per-material FP64 coefficients
+  (nu and the common scale 2 (lam + mu) / (pi mu (lam + 2 mu)) folded in,
+  src/mhs/kernels/image.py:95-104, 276-297, 405-406);
+* ``r_ddg`` / ``r_dg``: the R-family, straight-line code that lifts the
+  image-frame weighted moment tables W[n][u][v] to the Cartesian monomial
+  moments int N D1^a D2^b D3^c / R2^n (D = -xi1 e1' - xi2 e2' + z' n') and
+  accumulates the records into DDG (81) / DG (27);
+* ``q_ddg_point`` / ``q_dg_point``: the Q-family at one quadrature point.
+
+The traversal is depth-first over the monomial tree, so only one root-to-leaf
+path of expansion polynomials is live at a time.
+
+Run:  python -m mhs_amd.gen.emit
+"""
+from __future__ import annotations
+
+import json
+import pathlib
+from collections import defaultdict
+from fractions import Fraction
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+TABLE = ROOT / "data" / "mindlin_table.json"
+OUT = ROOT / "csrc" / "gen_image.inc"
+
+
+def tidx(u, v):
+    """Triangular index of the monomial xi1^u xi2^v."""
+    s = u + v
+    return s * (s + 1) // 2 + v
+
+
+def slot_of(idx):
+    s = 0
+    for i in idx:
+        s = s * 3 + i
+    return s
+
+
+class Emitter:
+    def __init__(self):
+        self.lines = []
+        self.n_coef = 0
+        self.coef_src = []          # (kind, list of (pnu, Fraction)) per coefficient
+
+    def coef(self, terms):
+        """Register one coefficient sum_t num/den nu^pnu; return its index."""
+        self.coef_src.append(terms)
+        self.n_coef += 1
+        return self.n_coef - 1
+
+    def w(self, s=""):
+        self.lines.append(s)
+
+
+def emit_r(em: Emitter, name, records, n_ix, n_slots):
+    """R-family: DDG (n_ix = 4) or DG (n_ix = 3)."""
+    # group records by monomial (a, b, c, n)
+    by_mono = defaultdict(list)
+    for rec in records:
+        slot = slot_of(rec[:n_ix])
+        a, b, c, n = rec[n_ix:n_ix + 4]
+        by_pz = defaultdict(list)
+        for pz, pnu, num, den in rec[n_ix + 4]:
+            by_pz[pz].append((pnu, Fraction(num, den)))
+        ks = {pz: em.coef(t) for pz, t in sorted(by_pz.items())}
+        by_mono[(a, b, c)].append((n, slot, ks))
+    # monomial tree: parent drops one factor (z first, then y, then x)
+    need = set(by_mono)
+    for m in list(need):
+        a, b, c = m
+        while a + b + c > 0:
+            if c > 0:
+                c -= 1
+            elif b > 0:
+                b -= 1
+            else:
+                a -= 1
+            need.add((a, b, c))
+
+    def parent(m):
+        a, b, c = m
+        if c > 0:
+            return (a, b, c - 1), 2
+        if b > 0:
+            return (a, b - 1, c), 1
+        return (a - 1, b, c), 0
+    children = defaultdict(list)
+    for m in need:
+        if m != (0, 0, 0):
+            children[parent(m)[0]].append(m)
+    em.w(f"__host__ __device__ __forceinline__ void {name}(const double* __restrict__ W3,")
+    em.w("    const double* __restrict__ W5, const double* __restrict__ W7,")
+    em.w("    const double* __restrict__ W9, const double* al, const double* be,")
+    em.w("    const double* ga, double z, const double* __restrict__ C, double* acc)")
+    em.w("{")
+    em.w("  const double z2 = z * z;")
+    Wn = {3: "W3", 5: "W5", 7: "W7", 9: "W9"}
+
+    def poly_name(m, u, v):
+        return f"p{m[0]}{m[1]}{m[2]}_{u}_{v}"
+
+    def visit(m, depth):
+        r = sum(m)
+        ind = "  " * (depth + 1)
+        if m == (0, 0, 0):
+            pass
+        else:
+            par, axis = parent(m)
+            for s in range(r + 1):
+                for v in range(s + 1):
+                    u = s - v
+                    terms = []
+                    if u > 0 and (u - 1) + v <= r - 1:
+                        terms.append(f"al[{axis}] * {poly_name(par, u - 1, v)}" if par != (0, 0, 0)
+                                     else (f"al[{axis}]" if (u - 1, v) == (0, 0) else None))
+                    if v > 0 and u + (v - 1) <= r - 1:
+                        terms.append(f"be[{axis}] * {poly_name(par, u, v - 1)}" if par != (0, 0, 0)
+                                     else (f"be[{axis}]" if (u, v - 1) == (0, 0) else None))
+                    if u + v <= r - 1:
+                        terms.append(f"ga[{axis}] * {poly_name(par, u, v)}" if par != (0, 0, 0)
+                                     else (f"ga[{axis}]" if (u, v) == (0, 0) else None))
+                    terms = [t for t in terms if t is not None]
+                    em.w(f"{ind}const double {poly_name(m, u, v)} = {' + '.join(terms) if terms else '0.0'};")
+        by_n = defaultdict(list)
+        for n, slot, ks in by_mono.get(m, []):
+            by_n[n].append((slot, ks))
+        for n in sorted(by_n):
+            if m == (0, 0, 0):
+                expr = f"{Wn[n]}[0]"
+            else:
+                expr = " + ".join(f"{poly_name(m, s - v, v)} * {Wn[n]}[{tidx(s - v, v)}]"
+                                  for s in range(r + 1) for v in range(s + 1))
+            em.w(f"{ind}{{ const double mm = {expr};")
+            em.w(f"{ind}  const double mz = z * mm, mzz = z2 * mm;")
+            for slot, ks in by_n[n]:
+                parts = []
+                for pz, k in ks.items():
+                    parts.append(f"C[{k}] * {('mm', 'mz', 'mzz')[pz]}")
+                em.w(f"{ind}  acc[{slot}] += {' + '.join(parts)};")
+            em.w(f"{ind}}}")
+        kids = sorted(children.get(m, []))
+        for ch in kids:
+            em.w(f"{ind}{{")
+            visit(ch, depth + 1)
+            em.w(f"{ind}}}")
+    visit((0, 0, 0), 0)
+    em.w("}")
+    em.w()
+
+
+def emit_q(em: Emitter, name, records, n_ix):
+    by_mono = defaultdict(list)
+    for rec in records:
+        slot = slot_of(rec[:n_ix])
+        a, b, c, n, q = rec[n_ix:n_ix + 5]
+        terms = [(pnu, Fraction(num, den)) for pz, pnu, num, den in rec[n_ix + 5]]
+        assert all(pz == 0 for pz, *_ in rec[n_ix + 5])
+        by_mono[(a, b, c, n, q)].append((slot, em.coef(terms)))
+    emax = [max(m[i] for m in by_mono) for i in range(5)]
+    em.w(f"__host__ __device__ __forceinline__ void {name}(double d1, double d2, double d3,")
+    em.w("    double ir, double iq, double w, const double* __restrict__ C, double* acc)")
+    em.w("{")
+    for nm, var, e in (("x", "d1", emax[0]), ("y", "d2", emax[1]), ("zz", "d3", emax[2]),
+                       ("r", "ir", emax[3]), ("q", "iq", emax[4])):
+        em.w(f"  const double {nm}1 = {var};")
+        for k in range(2, e + 1):
+            em.w(f"  const double {nm}{k} = {nm}{k - 1} * {var};")
+    dparts = sorted({m[:3] for m in by_mono})
+    rparts = sorted({m[3:] for m in by_mono})
+    for a, b, c in dparts:
+        f = ["w"] + [f"x{a}"] * (a > 0) + [f"y{b}"] * (b > 0) + [f"zz{c}"] * (c > 0)
+        em.w(f"  const double D{a}{b}{c} = {' * '.join(f)};")
+    for n, q in rparts:
+        f = [f"r{n}"] * (n > 0) + [f"q{q}"] * (q > 0)
+        em.w(f"  const double R{n}{q} = {' * '.join(f) if f else '1.0'};")
+    for m in sorted(by_mono):
+        a, b, c, n, q = m
+        em.w(f"  {{ const double mm = D{a}{b}{c} * R{n}{q};")
+        for slot, k in by_mono[m]:
+            em.w(f"    acc[{slot}] += C[{k}] * mm;")
+        em.w("  }")
+    em.w("}")
+    em.w()
+
+
+def main():
+    t = json.loads(TABLE.read_text())
+    em = Emitter()
+    em.w("// GENERATED by mhs_amd/gen/emit.py from mhs_amd/data/mindlin_table.json -- do not edit.")
+    em.w("#pragma once")
+    em.w()
+    body = Emitter()
+    body.coef_src = em.coef_src
+    offs = {}
+    offs["r_ddg"] = body.n_coef
+    emit_r(body, "r_ddg", t["r_ddg"], 4, 81)
+    offs["r_dg"] = body.n_coef = len(body.coef_src)
+    emit_r(body, "r_dg", t["r_dg"], 3, 27)
+    offs["q_ddg"] = body.n_coef = len(body.coef_src)
+    emit_q(body, "q_ddg_point", t["q_ddg"], 4)
+    offs["q_dg"] = body.n_coef = len(body.coef_src)
+    emit_q(body, "q_dg_point", t["q_dg"], 3)
+    n_coef = len(body.coef_src)
+    em.w(f"#define MHS_NCOEF {n_coef}")
+    for k, v in offs.items():
+        em.w(f"#define MHS_COEF_{k.upper()} {v}")
+    em.w()
+    # coefficient data: for each coefficient a list of (pnu, num, den)
+    flat = []
+    starts = [0]
+    for terms in body.coef_src:
+        for pnu, fr in terms:
+            flat.append((pnu, fr.numerator, fr.denominator))
+        starts.append(len(flat))
+    em.w(f"static const int mhs_coef_start[{len(starts)}] = {{{', '.join(map(str, starts))}}};")
+    em.w(f"static const double mhs_coef_terms[{len(flat)}][3] = {{")
+    for pnu, num, den in flat:
+        em.w(f"  {{{pnu}, {num}.0, {den}.0}},")
+    em.w("};")
+    em.w()
+    em.w("// Per-material coefficients: each record's sum num/den nu^pnu times the")
+    em.w("// common scale 2 (lam + mu) / (pi mu (lam + 2 mu)).  The R offsets index")
+    em.w("// the DDG block first, then DG; the Q offsets likewise.")
+    em.w("static inline void mhs_fill_coeffs(double mu, double lam, double* c)")
+    em.w("{")
+    em.w("  const double nu = lam / (2.0 * (lam + mu));")
+    em.w("  const double scale = 2.0 * (lam + mu) / (3.14159265358979323846 * mu * (lam + 2.0 * mu));")
+    em.w(f"  for (int k = 0; k < {n_coef}; ++k) {{")
+    em.w("    double s = 0.0;")
+    em.w("    for (int t = mhs_coef_start[k]; t < mhs_coef_start[k + 1]; ++t) {")
+    em.w("      double pw = 1.0;")
+    em.w("      for (int e = 0; e < (int)mhs_coef_terms[t][0]; ++e) pw *= nu;")
+    em.w("      s += (mhs_coef_terms[t][1] / mhs_coef_terms[t][2]) * pw;")
+    em.w("    }")
+    em.w("    c[k] = s * scale;")
+    em.w("  }")
+    em.w("}")
+    em.w()
+    em.lines += body.lines
+    OUT.write_text("\n".join(em.lines) + "\n")
+    print(f"wrote {OUT} ({len(em.lines)} lines, {n_coef} coefficients)")
+
+
+if __name__ == "__main__":
+    main()
