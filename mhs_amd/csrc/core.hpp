@@ -8,6 +8,21 @@
 #include <stdint.h>
 
 #define MHS_INLINE __host__ __device__ __forceinline__
+// The three largest blocks are real device calls, not inlined: fully inlined
+// into one pair kernel, hipcc (ROCm 7.2, -O2/-O3) produced lane-dependent
+// wrong results (NaN in one lane whenever the lanes of a wave took different
+// near/far/fold branches), while -O1, single-lane launches and any one of
+// these three as a call were correct.
+#define MHS_CALL __host__ __device__ __attribute__((noinline))
+#ifndef MHS_EDGE_FN
+#define MHS_EDGE_FN MHS_CALL
+#endif
+#ifndef MHS_NEAR_FN
+#define MHS_NEAR_FN MHS_CALL
+#endif
+#ifndef MHS_DIRECT_FN
+#define MHS_DIRECT_FN MHS_CALL
+#endif
 
 // ---------------------------------------------------------------- records --
 // Per-source record, MHS_SRC_STRIDE doubles (built on the host from the
@@ -109,7 +124,7 @@ template <int P> struct SpecI {      // image R-family: DEGREES {3:1, 5:3, 7:4, 
 // that equals the reference's expm1/direct forms to rounding.
 // Output E[mi][k], mi = (m + 1) / 2 over m = -1 .. 7.
 template <class S>
-MHS_INLINE void edge_integrals(double ua, double ub, double rho2, double E[5][8]) {
+MHS_EDGE_FN void edge_integrals(double ua, double ub, double rho2, double E[5][8]) {
   constexpr int KM = S::KMAX;
   const double rho = sqrt(rho2);
   const double aua = fabs(ua), aub = fabs(ub);
@@ -166,9 +181,7 @@ MHS_INLINE void edge_integrals(double ua, double ub, double rho2, double E[5][8]
         E[mi][k] = sc * acc[mi][k];
       }
     }
-    return;
-  }
-  if (same && umin >= 2.0 * rho) {
+  } else if (same && umin >= 2.0 * rho) {
     // large-|u| series on the positive edge (a, b) = (|.|min, |.|max):
     // a^s sum_j C(-m/2, j) x^j [t^q / q]_1^{b/a}, q = s - 2j, s = k - m + 1.
     const bool neg = ua < 0.0;
@@ -248,8 +261,7 @@ MHS_INLINE void edge_integrals(double ua, double ub, double rho2, double E[5][8]
         E[mi][k] = (neg && (k & 1)) ? -v : v;
       }
     }
-    return;
-  }
+  } else {
   // closed form (primitives.py:131-212): J_m, K_m and P_k^m = P_{k-2}^{m-2} - rho^2 P_{k-2}^m
   // over m = -3 .. 7 (index (m + 3) / 2).
   const double Ra = sqrt(ua * ua + rho2), Rb = sqrt(ub * ub + rho2);
@@ -326,6 +338,7 @@ MHS_INLINE void edge_integrals(double ua, double ub, double rho2, double E[5][8]
     for (int k = 0; k < 8; ++k)
       if (k <= S::EK[mi]) E[mi][k] = T[mi + 1][k];
   }
+  }
 }
 
 // --------------------------------------------------------- solid angle --
@@ -351,7 +364,7 @@ MHS_INLINE double solid_angle(const double* v, const double o[3]) {
 // Closed-form raw moments M_n^(a,b), n in DEG (moments.py:181-355).
 // M[ni][tix(a, b)], ni = (n + 1) / 2.
 template <class S>
-MHS_INLINE void raw_moments_near(const Frame& f, const double* __restrict__ v,
+MHS_NEAR_FN void raw_moments_near(const Frame& f, const double* __restrict__ v,
                                  double X1, double X2, double h2, double M[6][36]) {
   constexpr int KM = S::KMAX;
   const double h = sqrt(h2);
@@ -648,7 +661,7 @@ struct Mat {
 // Full-space TOTAL stress H[m][l][j] (global) and blob weight E of node k
 // (kernels.py:182-213), evaluated in the source's local frame where the
 // normal is e3, then rotated.
-MHS_INLINE void direct_H(const Frame& f, const double W[6][21], double z, double eps, const Mat& mt,
+MHS_DIRECT_FN void direct_H(const Frame& f, const double W[6][21], double z, double eps, const Mat& mt,
                          double H[27]) {
   const double nu = mt.nu, mu = mt.mu, lam = 2.0 * mu * nu / (1.0 - 2.0 * nu);
   const double C1 = 1.0 / (16.0 * 3.14159265358979323846 * mu * (1.0 - nu));
@@ -733,7 +746,7 @@ MHS_INLINE double direct_E(const double W[6][21], double eps) {
 }
 
 // Full-space displacement U[i][j] (global) of node k (kernels.py:148-163).
-MHS_INLINE void direct_U(const Frame& f, const double W[6][21], double z, double eps, const Mat& mt,
+MHS_DIRECT_FN void direct_U(const Frame& f, const double W[6][21], double z, double eps, const Mat& mt,
                          double U[9]) {
   const double nu = mt.nu, mu = mt.mu, lam = 2.0 * mu * nu / (1.0 - 2.0 * nu);
   const double C1 = 1.0 / (16.0 * 3.14159265358979323846 * mu * (1.0 - nu));
