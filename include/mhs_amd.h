@@ -37,6 +37,11 @@ enum { MHS_DISP = 0, MHS_STRESS = 1, MHS_EIGEN = 2, MHS_TRACTION = 3, MHS_INTERA
  * (subtract_eigenstress=True), also write the eigenstress to out2 (parts=True). */
 enum { MHS_FOLD = 1, MHS_SUBTRACT = 2, MHS_PARTS = 4 };
 
+/* Passes of one assembly: the pair pass writes every entry (direct + image
+ * R-family or folded mirror + eigenstress), the Q-family and the Boussinesq
+ * fold passes add to it.  A partial mask is for timing a pass alone. */
+enum { MHS_STAGE_PAIR = 1, MHS_STAGE_Q = 2, MHS_STAGE_FOLD = 4 };
+
 typedef struct mhs_launch {
   int32_t abi_version;      /* MHS_ABI_VERSION */
   int32_t kind;             /* MHS_DISP .. MHS_INTERACTION */
@@ -62,6 +67,8 @@ typedef struct mhs_launch {
   int32_t* fold_count;      /* scratch: 2 int32 */
   int32_t* fold_err;        /* 1 int32: bit 0 grid overflow, bit 1 list overflow (caller zeroes) */
   int64_t fold_cap;
+  int32_t stages;           /* 0 = all passes; else a mask of MHS_STAGE_* (timing, serving) */
+  int32_t reserved;
 } mhs_launch;
 
 /* Assemble one matrix block: replaces the per-source loops of
@@ -80,6 +87,12 @@ int mhs_assemble(const mhs_launch* launch, void* stream);
 int mhs_gather(const double* scratch, double* out, int64_t n_lead, int64_t n_elem_cols,
                int64_t n_nodes, int tail, const int64_t* ptr, const int64_t* idx,
                const double* sb, int n_a, int n_b, void* stream);
+
+/* Branch histogram of the launch's pairs (adds into 13 device uint64
+ * counters): pairs; direct near / far-12^2 / far-8^2; image near / far12 /
+ * far8; folded pairs; Q-family points and pairs; folded mirror near / far12 /
+ * far8.  The work model of the roofline figures. */
+int mhs_histogram(const mhs_launch* launch, unsigned long long* counts, void* stream);
 
 /* Elastic strain from elastic stress in place (matrices.py:572-579). */
 int mhs_strain(double* buf, int64_t n_obs, int64_t n_cols, double mu, double lam, void* stream);

@@ -14,6 +14,7 @@ ABI_VERSION = 1
 
 KIND = {"disp": 0, "stress": 1, "eigen": 2, "traction": 3, "interaction": 4}
 F_FOLD, F_SUBTRACT, F_PARTS = 1, 2, 4
+STAGE_PAIR, STAGE_Q, STAGE_FOLD = 1, 2, 4
 
 
 class Launch(C.Structure):
@@ -29,13 +30,13 @@ class Launch(C.Structure):
         ("out", C.c_void_p), ("out2", C.c_void_p),
         ("strad", C.c_void_p), ("n_strad", C.c_int64),
         ("fold_list", C.c_void_p), ("fold_count", C.c_void_p), ("fold_err", C.c_void_p),
-        ("fold_cap", C.c_int64),
+        ("fold_cap", C.c_int64), ("stages", C.c_int32), ("reserved", C.c_int32),
     ]
 
 
 _LIB = None
 
-EXPORTS = ("mhs_assemble", "mhs_gather", "mhs_strain", "mhs_fill_material", "mhs_ncoef",
+EXPORTS = ("mhs_assemble", "mhs_histogram", "mhs_gather", "mhs_strain", "mhs_fill_material", "mhs_ncoef",
            "mhs_src_stride", "mhs_abi_version", "mhs_last_error")
 
 
@@ -52,6 +53,8 @@ def load(path=None):
     lib = C.CDLL(str(p))
     lib.mhs_assemble.argtypes = [C.POINTER(Launch), C.c_void_p]
     lib.mhs_assemble.restype = C.c_int
+    lib.mhs_histogram.argtypes = [C.POINTER(Launch), C.c_void_p, C.c_void_p]
+    lib.mhs_histogram.restype = C.c_int
     lib.mhs_gather.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_int,
                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
     lib.mhs_gather.restype = C.c_int

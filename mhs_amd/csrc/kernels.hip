@@ -183,6 +183,47 @@ __global__ void __launch_bounds__(256) k_fold(Params pr, const int64_t* __restri
   }
 }
 
+
+// ------------------------------------------------------------- histogram --
+// Branch histogram of the pair set, with the kernels' own classification
+// (the work model behind the roofline figures).  counts[0..12]:
+// pairs, direct near / far12 / far8, image near / far12 / far8, fold pairs,
+// Q points, Q pairs, mirror near / far12 / far8.
+__global__ void k_hist(Params pr, int order, unsigned long long* __restrict__ counts) {
+  __shared__ unsigned long long acc[13];
+  if (threadIdx.x < 13) acc[threadIdx.x] = 0ull;
+  __syncthreads();
+  const int K = (order + 1) * (order + 2) / 2;
+  const int64_t nblk = (pr.n_cols + 255) / 256;
+  const int64_t o = pr.obs_lo + blockIdx.x / nblk;
+  const int64_t col = (blockIdx.x % nblk) * 256 + threadIdx.x;
+  if (col < pr.n_cols) {
+    const int64_t s = col / K;
+    const double* src = pr.src + s * MHS_SRC_STRIDE;
+    const double ob[3] = {pr.obs[3 * o], pr.obs[3 * o + 1], pr.obs[3 * o + 2]};
+    const bool band = in_fold_band(src, ob, pr.flags);
+    const double eps = src[S_EPS], L = src[S_L];
+    atomicAdd(&acc[0], 1ull);
+    for (int img = 0; img < 2; ++img) {
+      const int oc = img ? S_IC : S_C;
+      const double d0 = ob[0] - src[oc], d1 = ob[1] - src[oc + 1], d2 = ob[2] - src[oc + 2];
+      const double D = sqrt(d0 * d0 + d1 * d1 + d2 * d2 + eps * eps);
+      const int cls = D <= 10.0 * L ? 0 : (D > 40.0 * L ? 2 : 1);
+      const int base = img ? (band ? 10 : 4) : 1;
+      atomicAdd(&acc[base + cls], 1ull);
+    }
+    if (band) {
+      atomicAdd(&acc[7], 1ull);
+    } else {
+      const int nq = order == 0 ? q_order<0>(src, ob) : (order == 1 ? q_order<1>(src, ob) : q_order<2>(src, ob));
+      atomicAdd(&acc[8], (unsigned long long)(nq * nq));
+      atomicAdd(&acc[9], 1ull);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 13) atomicAdd(&counts[threadIdx.x], acc[threadIdx.x]);
+}
+
 // ---------------------------------------------------------------- gather --
 // Continuous layout: out[..., g, ...] = sum over the element columns c of
 // shared node g (CSR, ascending) of scratch[..., c, ...].  Interaction
@@ -243,15 +284,20 @@ static int launch_all(const mhs_launch* L, const Params& pr, hipStream_t st) {
   if (blocks > 0x7fffffffLL) return fail("mhs_assemble: launch too large; split the observer rows");
   const dim3 grid((unsigned)blocks), block(256);
   const bool stress = pr.kind != K_DISP;
-  if (pr.kind == K_DISP) hipLaunchKernelGGL((k_pair<P, 1>), grid, block, 0, st, pr);
-  else if (pr.kind == K_EIGEN) hipLaunchKernelGGL((k_pair<P, 2>), grid, block, 0, st, pr);
-  else hipLaunchKernelGGL((k_pair<P, 0>), grid, block, 0, st, pr);
-  if (check(hipGetLastError(), "k_pair launch")) return -1;
+  const int stages = L->stages ? L->stages : (MHS_STAGE_PAIR | MHS_STAGE_Q | MHS_STAGE_FOLD);
+  if (stages & MHS_STAGE_PAIR) {
+    if (pr.kind == K_DISP) hipLaunchKernelGGL((k_pair<P, 1>), grid, block, 0, st, pr);
+    else if (pr.kind == K_EIGEN) hipLaunchKernelGGL((k_pair<P, 2>), grid, block, 0, st, pr);
+    else hipLaunchKernelGGL((k_pair<P, 0>), grid, block, 0, st, pr);
+    if (check(hipGetLastError(), "k_pair launch")) return -1;
+  }
   if (pr.kind == K_EIGEN) return 0;
-  if (stress) hipLaunchKernelGGL((k_q<P, true>), grid, block, 0, st, pr);
-  else hipLaunchKernelGGL((k_q<P, false>), grid, block, 0, st, pr);
-  if (check(hipGetLastError(), "k_q launch")) return -1;
-  if ((pr.flags & F_FOLD) && L->n_strad > 0) {
+  if (stages & MHS_STAGE_Q) {
+    if (stress) hipLaunchKernelGGL((k_q<P, true>), grid, block, 0, st, pr);
+    else hipLaunchKernelGGL((k_q<P, false>), grid, block, 0, st, pr);
+    if (check(hipGetLastError(), "k_q launch")) return -1;
+  }
+  if ((stages & MHS_STAGE_FOLD) && (pr.flags & F_FOLD) && L->n_strad > 0) {
     if (!L->fold_list || !L->fold_count || !L->fold_err) return fail("mhs_assemble: fold buffers missing");
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, L->fold_count, 2);
     const int64_t nt = n_rows * L->n_strad;
@@ -337,6 +383,18 @@ int mhs_strain(double* buf, int64_t n_obs, int64_t n_cols, double mu, double lam
   hipLaunchKernelGGL(k_strain, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, (hipStream_t)stream, buf,
                      n_obs, n_cols, mu, lam);
   return check(hipGetLastError(), "k_strain launch");
+}
+
+int mhs_histogram(const mhs_launch* L, unsigned long long* counts, void* stream) {
+  if (!L || !counts || !L->obs || !L->src) return fail("mhs_histogram: null pointer");
+  Params pr{};
+  pr.obs = L->obs; pr.src = L->src; pr.n_obs = L->n_obs; pr.n_src = L->n_src; pr.n_cols = L->n_cols;
+  pr.obs_lo = L->obs_lo; pr.flags = L->flags;
+  const int64_t blocks = (L->obs_hi - L->obs_lo) * ((L->n_cols + 255) / 256);
+  if (blocks <= 0) return 0;
+  if (blocks > 0x7fffffffLL) return fail("mhs_histogram: too large");
+  hipLaunchKernelGGL(k_hist, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, pr, L->order, counts);
+  return check(hipGetLastError(), "k_hist launch");
 }
 
 }  // extern "C"

@@ -1,0 +1,124 @@
+"""Algorithmic work of an assembly: the branch histogram and FLOP model.
+
+Every pair takes branches the reference's rules fix exactly, so the point
+counts are computable on the host without running the kernels:
+
+* direct term: closed form if sqrt(|x - c|^2 + eps^2) <= 10 L, else Gauss
+  12^2 (<= 40 L) or 8^2 points (moments.py:551-554, 589-601);
+* image R-family: the same rule on the reflected triangle (image.py:458-464),
+  unless the pair is folded (fold band, fold.py:84-118), where the mirror
+  element's full-space field takes its place;
+* image Q-family: n_q^2 points, n_q = clip(ceil(10 L / sqrt(delta^2 + eps^2))
+  + p, 8, 192) with delta the distance to the reflected triangle
+  (image.py:211-254); none for a folded pair.
+
+FLOPs per branch are static counts of this repository's kernel source
+(csrc/core.hpp, pair.hpp, gen_image.inc), FMA = 2, add/mul/div/sqrt/rcp = 1,
+at order P0 in the stress family (the interaction workload):
+
+* far direct, per Gauss point: barycentric point and weight 14, offset and
+  R^2 7, 1/sqrt 2, R^-3, R^-5, R^-7 4, in-plane monomials to degree 4 14,
+  22 table accumulations 44                                     -> 85
+* far image R-family, per Gauss point: 28 + monomials to degree 5 20 +
+  R^-3..R^-9 5 + 49 table accumulations 98                      -> 151
+* Q-family, per Gauss point: point, weight, offsets, R2, 1/R2, 1/Q 32;
+  power ladders 18; 35 D-monomials (with the weight) 60; 32 radial parts 32;
+  236 monomials 236; 1134 record FMAs 2268                       -> 2646
+* per pair, closed-form near direct (three edges, solid angle, recursion)
+  ~2.6e3; near image ~5.2e3; direct lift + contraction + rotation ~3.0e3;
+  R-family lift + records + contraction ~6.0e3; epilogue 2e2.
+
+The near-field constants are averages over the edge regimes; the quadrature
+ones are exact per point.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import defaults, geometry
+
+FLOP_DIRECT_PT = 85
+FLOP_IMAGE_PT = 151
+FLOP_Q_PT = 2646
+FLOP_DIRECT_NEAR = 2600
+FLOP_IMAGE_NEAR = 5200
+FLOP_DIRECT_LIFT = 3000
+FLOP_IMAGE_LIFT = 6000
+FLOP_EPILOGUE = 200
+
+
+def _tri_dist(obs, v):
+    """Vectorised point-triangle distance, obs (N, 3), v (S, 3, 3) -> (N, S)."""
+    a, b, c = v[:, 0], v[:, 1], v[:, 2]
+    ab, ac = b - a, c - a
+    ap = obs[:, None, :] - a[None]
+    d20 = np.einsum("nsi,si->ns", ap, ab)
+    d21 = np.einsum("nsi,si->ns", ap, ac)
+    d00 = np.einsum("si,si->s", ab, ab)
+    d01 = np.einsum("si,si->s", ab, ac)
+    d11 = np.einsum("si,si->s", ac, ac)
+    den = d00 * d11 - d01 * d01
+    vv = (d11 * d20 - d01 * d21) / den
+    ww = (d00 * d21 - d01 * d20) / den
+    uu = 1.0 - vv - ww
+    nrm = np.cross(ab, ac)
+    plane = np.abs(np.einsum("nsi,si->ns", ap, nrm)) / np.linalg.norm(nrm, axis=1)
+    best = np.full(plane.shape, np.inf)
+    for p0, p1 in ((a, b), (b, c), (c, a)):
+        e = p1 - p0
+        t = np.clip(np.einsum("nsi,si->ns", obs[:, None, :] - p0[None], e) / np.einsum("si,si->s", e, e), 0, 1)
+        q = p0[None] + t[..., None] * e[None]
+        best = np.minimum(best, np.linalg.norm(obs[:, None, :] - q, axis=2))
+    inside = (uu >= 0) & (vv >= 0) & (ww >= 0)
+    return np.where(inside, plane, best)
+
+
+def histogram(obs, tris, eps, order=0, fold=True, chunk=256) -> dict:
+    """Branch counts over all (observer, source) pairs."""
+    obs = np.asarray(obs, float)
+    recs = geometry.source_records(tris, np.broadcast_to(eps, (tris.shape[0],)))
+    c = recs[:, geometry.S_C:geometry.S_C + 3]
+    ic = recs[:, geometry.S_IC:geometry.S_IC + 3]
+    L = recs[:, geometry.S_L]
+    e = recs[:, geometry.S_EPS]
+    vi = np.asarray(tris, float) * np.array([1.0, 1.0, -1.0])
+    strad = recs[:, geometry.S_STRADDLE] != 0.0
+    crease = recs[:, geometry.S_CREASE:geometry.S_CREASE + 2]
+    h = dict(pairs=0, direct_near=0, direct_far12=0, direct_far8=0, image_near=0,
+             image_far12=0, image_far8=0, fold_pairs=0, q_points=0, q_pairs=0)
+    for lo in range(0, obs.shape[0], chunk):
+        o = obs[lo:lo + chunk]
+        band = np.zeros((o.shape[0], tris.shape[0]), bool)
+        if fold:
+            reach = np.hypot(o[:, None, 0] - crease[None, :, 0], o[:, None, 1] - crease[None, :, 1])
+            band = (strad[None] & (-o[:, None, 2] < defaults.FOLD_BAND_DEPTH_EPS * e[None])
+                    & (reach < defaults.FOLD_REACH_EPS * e[None]))
+        for cen, key in ((c, "direct"), (ic, "image")):
+            D = np.sqrt(np.sum((o[:, None, :] - cen[None]) ** 2, axis=2) + e[None] ** 2)
+            near = D <= defaults.D_STAR * L[None]
+            far8 = D > defaults.D_STAR_DISTANT * L[None]
+            sel = ~band if key == "image" else np.ones_like(band)
+            h[f"{key}_near"] += int(np.sum(near & sel))
+            h[f"{key}_far8"] += int(np.sum(~near & far8 & sel))
+            h[f"{key}_far12"] += int(np.sum(~near & ~far8 & sel))
+        delta = _tri_dist(o, vi)
+        nq = np.clip(np.ceil(defaults.IMAGE_Q_BUDGET_C * L[None] / np.sqrt(delta ** 2 + e[None] ** 2)) + order,
+                     defaults.IMAGE_Q_GAUSS_MIN, defaults.IMAGE_Q_GAUSS_MAX)
+        h["q_points"] += int(np.sum(np.where(band, 0, nq * nq)))
+        h["q_pairs"] += int(np.sum(~band))
+        h["fold_pairs"] += int(np.sum(band))
+        h["pairs"] += band.size
+    return h
+
+
+def flops(h: dict) -> dict:
+    """FLOPs per pass from a histogram (P0, stress family)."""
+    pair = (h["direct_near"] * FLOP_DIRECT_NEAR + h["direct_far12"] * 144 * FLOP_DIRECT_PT
+            + h["direct_far8"] * 64 * FLOP_DIRECT_PT + h["pairs"] * (FLOP_DIRECT_LIFT + FLOP_EPILOGUE)
+            + h["image_near"] * FLOP_IMAGE_NEAR + h["image_far12"] * 144 * FLOP_IMAGE_PT
+            + h["image_far8"] * 64 * FLOP_IMAGE_PT
+            + h.get("mirror_near", 0) * FLOP_DIRECT_NEAR + h.get("mirror_far12", 0) * 144 * FLOP_DIRECT_PT
+            + h.get("mirror_far8", 0) * 64 * FLOP_DIRECT_PT + h["fold_pairs"] * FLOP_DIRECT_LIFT
+            + (h["pairs"] - h["fold_pairs"]) * FLOP_IMAGE_LIFT)
+    q = h["q_points"] * FLOP_Q_PT + h["q_pairs"] * 500
+    return {"pair": float(pair), "q": float(q)}

@@ -131,88 +131,137 @@ def _csr(ids: np.ndarray, n_nodes: int):
     return ptr, order.astype(np.int64)
 
 
+class Prepared:
+    """A job with its inputs resident in HBM and its outputs allocated: the
+    host-side planning happens here, once; :meth:`launch` only enqueues
+    kernels (what a benchmark times and what the serving engine replays)."""
+
+    def __init__(self, job: Job, stream=None, dev: Device | None = None):
+        dev = dev or device()
+        self.dev, self.job = dev, job
+        torch = dev.torch
+        self.stream = stream if stream is not None else torch.cuda.current_stream(dev.dev)
+        self.handle = C.c_void_p(self.stream.cuda_stream)
+        K = job.K
+        self.n_cols = n_cols = job.n_src * K
+        recs = geometry.source_records(job.tris, job.eps)
+        strad = (np.flatnonzero(recs[:, geometry.S_STRADDLE] != 0.0).astype(np.int64)
+                 if job.fold else np.zeros(0, np.int64))
+        self.n_strad = len(strad)
+        self.cont = cont = job.ids is not None
+        with torch.cuda.stream(self.stream):
+            self.obs_t = dev.put(job.obs)
+            self.src_t = dev.put(recs)
+            self.nrm_t = dev.put(job.nrm)
+            self.rb_t = dev.put(job.rb)
+            self.sb_t = None if cont else dev.put(job.sb)
+            self.strad_t = dev.put(strad, np.int64) if strad.size else None
+            self.coef_t = dev.coef(job.mu, job.lam)
+            self.final = torch.empty(job.out_shape(), dtype=torch.float64, device=dev.dev)
+            self.final2 = torch.empty_like(self.final) if job.parts else None
+            if cont:
+                elem_shape = list(job.out_shape(n_cols))
+                if job.kind == "interaction":
+                    elem_shape[-1] = 3
+                self.elem_shape = elem_shape
+                row_bytes = 8 * int(np.prod(elem_shape[1:])) * (2 if job.parts else 1)
+                self.rows = max(1, min(job.n_obs, ROW_SCRATCH_BYTES // max(row_bytes, 1)))
+                ptr, idx = _csr(job.ids, job.n_nodes)
+                self.ptr_t, self.idx_t = dev.put(ptr, np.int64), dev.put(idx, np.int64)
+                self.sbn_t = dev.put(job.sb) if job.kind == "interaction" else None
+                scr_shape = [min(self.rows, job.n_obs)] + elem_shape[1:]
+                self.scr = torch.empty(scr_shape, dtype=torch.float64, device=dev.dev)
+                self.scr2 = torch.empty_like(self.scr) if job.parts else None
+            else:
+                self.rows = job.n_obs
+            self.fold_cap = max(1, self.rows * self.n_strad)
+            self.fold_list = (torch.empty(2 * self.fold_cap, dtype=torch.int64, device=dev.dev)
+                              if strad.size else None)
+            self.fold_cnt = torch.zeros(2, dtype=torch.int32, device=dev.dev)
+            self.fold_err = torch.zeros(1, dtype=torch.int32, device=dev.dev)
+        self.kind = _lib.KIND[job.kind]
+        self.flags = ((_lib.F_FOLD if job.fold else 0) | (_lib.F_SUBTRACT if job.subtract else 0)
+                      | (_lib.F_PARTS if job.parts else 0))
+        self.n_b_launch = 3 if (cont and job.kind == "interaction") else job.n_b
+
+    def _launch_struct(self, lo, hi, out_p, out2_p, stages):
+        job, dev = self.job, self.dev
+        return _lib.Launch(
+            abi_version=_lib.ABI_VERSION, kind=self.kind, order=job.order, flags=self.flags,
+            n_a=job.n_a, n_b=self.n_b_launch, n_obs=job.n_obs, n_src=job.n_src, n_cols=self.n_cols,
+            obs_lo=lo, obs_hi=hi, mu=job.mu, lam=job.lam,
+            obs=_ptr(self.obs_t), nrm=_ptr(self.nrm_t), rb=_ptr(self.rb_t), sb=_ptr(self.sb_t),
+            src=_ptr(self.src_t), coef=_ptr(self.coef_t), gl_x=_ptr(dev.gl_x), gl_w=_ptr(dev.gl_w),
+            out=out_p, out2=out2_p, strad=_ptr(self.strad_t), n_strad=self.n_strad,
+            fold_list=_ptr(self.fold_list), fold_count=_ptr(self.fold_cnt), fold_err=_ptr(self.fold_err),
+            fold_cap=self.fold_cap, stages=stages)
+
+    def launch(self, stages: int = 0):
+        """Enqueue every pass on the stream (no host synchronisation)."""
+        job, lib = self.job, self.dev.lib
+        for lo in range(0, job.n_obs, self.rows):
+            hi = min(job.n_obs, lo + self.rows)
+            if self.cont:
+                out_p, out2_p = _ptr(self.scr), _ptr(self.scr2)
+            else:
+                row_elems = int(np.prod(self.final.shape[1:]))
+                out_p = C.c_void_p(self.final.data_ptr() + 8 * lo * row_elems)
+                out2_p = (C.c_void_p(self.final2.data_ptr() + 8 * lo * row_elems)
+                          if job.parts else None)
+            L = self._launch_struct(lo, hi, out_p, out2_p, stages)
+            _lib.check(lib.mhs_assemble(C.byref(L), self.handle), lib)
+            if self.cont:
+                self._gather(lo, hi)
+        if job.strain:
+            _lib.check(lib.mhs_strain(_ptr(self.final), job.n_obs, job.n_dof, job.mu, job.lam,
+                                      self.handle), lib)
+
+    def _gather(self, lo, hi):
+        job, lib = self.job, self.dev.lib
+        for s_t, f_t in ((self.scr, self.final), (self.scr2, self.final2)):
+            if s_t is None:
+                continue
+            dst = C.c_void_p(f_t.data_ptr() + 8 * lo * int(np.prod(f_t.shape[1:])))
+            if job.kind == "interaction":
+                rc = lib.mhs_gather(_ptr(s_t), dst, hi - lo, self.n_cols, job.n_nodes, 0,
+                                    _ptr(self.ptr_t), _ptr(self.idx_t), _ptr(self.sbn_t),
+                                    job.n_a, job.n_b, self.handle)
+            else:
+                lead = (hi - lo) * int(np.prod(s_t.shape[1:-2]))
+                rc = lib.mhs_gather(_ptr(s_t), dst, lead, self.n_cols, job.n_nodes, 3,
+                                    _ptr(self.ptr_t), _ptr(self.idx_t), None, 0, 0, self.handle)
+            _lib.check(rc, lib)
+
+    HIST_KEYS = ("pairs", "direct_near", "direct_far12", "direct_far8", "image_near", "image_far12",
+                 "image_far8", "fold_pairs", "q_points", "q_pairs", "mirror_near", "mirror_far12",
+                 "mirror_far8")
+
+    def histogram(self) -> dict:
+        """Branch counts of every pair, by the kernels' own classification."""
+        torch, lib = self.dev.torch, self.dev.lib
+        with torch.cuda.stream(self.stream):
+            cnt = torch.zeros(13, dtype=torch.int64, device=self.dev.dev)
+            L = self._launch_struct(0, self.job.n_obs, None, None, 0)
+            _lib.check(lib.mhs_histogram(C.byref(L), _ptr(cnt), self.handle), lib)
+            vals = cnt.cpu().tolist()
+        return dict(zip(self.HIST_KEYS, vals))
+
+    def check_fold(self):
+        """Raise if a folded pair overflowed its grid or the pair list (syncs)."""
+        if self.n_strad:
+            err = int(self.fold_err.cpu()[0])
+            if err:
+                raise RuntimeError(f"mhs_amd: fold grid/list overflow (flags {err})")
+
+
 def run(job: Job, stream=None, dev: Device | None = None):
     """Execute ``job`` on the GPU; returns (out, out2) device tensors in the
     final layout (out2 is the eigenstress part for parts=True, else None)."""
-    dev = dev or device()
-    torch, lib = dev.torch, dev.lib
-    st = stream if stream is not None else torch.cuda.current_stream(dev.dev)
-    handle = C.c_void_p(st.cuda_stream)
-    K = job.K
-    n_cols = job.n_src * K
-    recs = geometry.source_records(job.tris, job.eps)
-    strad = np.flatnonzero(recs[:, geometry.S_STRADDLE] != 0.0).astype(np.int64) if job.fold else np.zeros(0, np.int64)
-    with torch.cuda.stream(st):
-        obs_t = dev.put(job.obs)
-        src_t = dev.put(recs)
-        nrm_t = dev.put(job.nrm)
-        rb_t = dev.put(job.rb)
-        cont = job.ids is not None
-        sb_t = None if cont else dev.put(job.sb)
-        strad_t = dev.put(strad, np.int64) if strad.size else None
-        coef_t = dev.coef(job.mu, job.lam)
-        final = torch.empty(job.out_shape(), dtype=torch.float64, device=dev.dev)
-        final2 = torch.empty_like(final) if job.parts else None
-        kind = _lib.KIND[job.kind]
-        flags = ((_lib.F_FOLD if job.fold else 0) | (_lib.F_SUBTRACT if job.subtract else 0)
-                 | (_lib.F_PARTS if job.parts else 0))
-        n_b_launch = 3 if (cont and job.kind == "interaction") else job.n_b
-        if cont:
-            elem_shape = list(job.out_shape(n_cols))
-            if job.kind == "interaction":
-                elem_shape[-1] = 3
-            row_bytes = 8 * int(np.prod(elem_shape[1:])) * (2 if job.parts else 1)
-            rows = max(1, min(job.n_obs, ROW_SCRATCH_BYTES // max(row_bytes, 1)))
-            ptr, idx = _csr(job.ids, job.n_nodes)
-            ptr_t, idx_t = dev.put(ptr, np.int64), dev.put(idx, np.int64)
-            sbn_t = dev.put(job.sb) if job.kind == "interaction" else None
-        else:
-            rows = job.n_obs
-        fold_cap = max(1, rows * len(strad))
-        fold_list = torch.empty(2 * fold_cap, dtype=torch.int64, device=dev.dev) if strad.size else None
-        fold_cnt = torch.zeros(2, dtype=torch.int32, device=dev.dev)
-        fold_err = torch.zeros(1, dtype=torch.int32, device=dev.dev)
-        for lo in range(0, job.n_obs, rows):
-            hi = min(job.n_obs, lo + rows)
-            if cont:
-                scr_shape = [hi - lo] + elem_shape[1:]
-                scr = torch.empty(scr_shape, dtype=torch.float64, device=dev.dev)
-                scr2 = torch.empty_like(scr) if job.parts else None
-                out_p, out2_p = _ptr(scr), _ptr(scr2)
-            else:
-                row_elems = int(np.prod(final.shape[1:]))
-                out_p = C.c_void_p(final.data_ptr() + 8 * lo * row_elems)
-                out2_p = C.c_void_p(final2.data_ptr() + 8 * lo * row_elems) if job.parts else None
-            L = _lib.Launch(
-                abi_version=_lib.ABI_VERSION, kind=kind, order=job.order, flags=flags,
-                n_a=job.n_a, n_b=n_b_launch, n_obs=job.n_obs, n_src=job.n_src, n_cols=n_cols,
-                obs_lo=lo, obs_hi=hi, mu=job.mu, lam=job.lam,
-                obs=_ptr(obs_t), nrm=_ptr(nrm_t), rb=_ptr(rb_t), sb=_ptr(sb_t), src=_ptr(src_t),
-                coef=_ptr(coef_t), gl_x=_ptr(dev.gl_x), gl_w=_ptr(dev.gl_w),
-                out=out_p, out2=out2_p, strad=_ptr(strad_t), n_strad=len(strad),
-                fold_list=_ptr(fold_list), fold_count=_ptr(fold_cnt), fold_err=_ptr(fold_err),
-                fold_cap=fold_cap)
-            _lib.check(lib.mhs_assemble(C.byref(L), handle), lib)
-            if cont:
-                for s_t, f_t in ((scr, final), (scr2, final2)):
-                    if s_t is None:
-                        continue
-                    dst = C.c_void_p(f_t.data_ptr() + 8 * lo * int(np.prod(f_t.shape[1:])))
-                    if job.kind == "interaction":
-                        rc = lib.mhs_gather(_ptr(s_t), dst, hi - lo, n_cols, job.n_nodes, 0, _ptr(ptr_t),
-                                            _ptr(idx_t), _ptr(sbn_t), job.n_a, job.n_b, handle)
-                    else:
-                        lead = (hi - lo) * int(np.prod(s_t.shape[1:-2]))
-                        rc = lib.mhs_gather(_ptr(s_t), dst, lead, n_cols, job.n_nodes, 3, _ptr(ptr_t),
-                                            _ptr(idx_t), None, 0, 0, handle)
-                    _lib.check(rc, lib)
-        if job.strain:
-            _lib.check(lib.mhs_strain(_ptr(final), job.n_obs, job.n_dof, job.mu, job.lam, handle), lib)
-        err = int(fold_err.cpu()[0]) if strad.size else 0
-    if err:
-        raise RuntimeError(f"mhs_amd: fold grid/list overflow (flags {err}); the polar grid of a "
-                           f"folded pair exceeded its panel budget")
-    return final, final2
+    prep = Prepared(job, stream, dev)
+    with prep.dev.torch.cuda.stream(prep.stream):
+        prep.launch()
+    prep.check_fold()
+    return prep.final, prep.final2
 
 
 def to_host(t, out=None):
