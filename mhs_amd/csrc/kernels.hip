@@ -53,6 +53,98 @@ __global__ void __launch_bounds__(256) k_q(Params pr) {
   q_item<P, STRESS>(pr, o, col);
 }
 
+
+// ------------------------------------------------- contracted Q-family --
+// Interaction with one or two output components: the Q-family of a pair
+// collapses to sum_k kappa_k * monomial_k, kappa = the records contracted with
+// the pair's output functional G[slot] = E_a[i][p] F_b[j][m] (receiver and
+// source halves of the traction-operator / Hooke / frame chain).  One wave per
+// pair: lanes share out the quadrature points and kappa lives in LDS.
+template <int P, int NOUT>
+__global__ void __launch_bounds__(256) k_qc(Params pr, int64_t n_items) {
+  constexpr int K = (P + 1) * (P + 2) / 2;
+  __shared__ double kap_s[4][NOUT * MHS_QM];
+  __shared__ double g_s[4][NOUT * 81];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t item = (int64_t)blockIdx.x * 4 + wv;
+  bool valid = item < n_items;
+  int64_t o = 0, col = 0, s = 0;
+  int k = 0, nq = 0;
+  const double* src = pr.src;
+  double ob[3] = {0.0, 0.0, 0.0};
+  if (valid) {
+    o = pr.obs_lo + item / pr.n_cols;
+    col = item % pr.n_cols;
+    s = col / K;
+    k = (int)(col - s * K);
+    src = pr.src + s * MHS_SRC_STRIDE;
+    ob[0] = pr.obs[3 * o]; ob[1] = pr.obs[3 * o + 1]; ob[2] = pr.obs[3 * o + 2];
+    valid = !in_fold_band(src, ob, pr.flags);
+    if (valid) nq = q_order<P>(src, ob);
+  }
+  // output functional G[out][slot] = E_a[i][p] * F_b[j][m]
+  if (valid) {
+    const double* nr = pr.nrm + 3 * o;
+    const double* n = src + S_N;
+    for (int t = lane; t < NOUT * 81; t += 64) {
+      const int out = t / 81, slot = t % 81;
+      const int a = out / pr.n_b, b = out % pr.n_b;
+      const double* rb = pr.rb + ((int64_t)o * pr.n_a + a) * 3;
+      const double* sb = pr.sb + ((int64_t)s * pr.n_b + b) * 3;
+      const int i = slot / 27, j = (slot / 9) % 3, p = (slot / 3) % 3, m = slot % 3;
+      const double rn = rb[0] * nr[0] + rb[1] * nr[1] + rb[2] * nr[2];
+      const double sn = sb[0] * n[0] + sb[1] * n[1] + sb[2] * n[2];
+      const double E = pr.mu * (rb[i] * nr[p] + rb[p] * nr[i]) + (i == p ? pr.lam * rn : 0.0);
+      const double F = pr.mu * (sb[j] * n[m] + n[j] * sb[m]) + (j == m ? pr.lam * sn : 0.0);
+      g_s[wv][t] = E * F;
+    }
+  }
+  __syncthreads();
+  if (valid) {
+    for (int t = lane; t < NOUT * MHS_QM; t += 64) {
+      const int out = t / MHS_QM, km = t % MHS_QM;
+      double acc = 0.0;
+      for (int r = mhs_qm_ptr[km]; r < mhs_qm_ptr[km + 1]; ++r)
+        acc = fma(pr.coef[mhs_qm_coef[r]], g_s[wv][out * 81 + mhs_qm_slot[r]], acc);
+      kap_s[wv][t] = acc;
+    }
+  }
+  __syncthreads();
+  double acc[NOUT];
+#pragma unroll
+  for (int t = 0; t < NOUT; ++t) acc[t] = 0.0;
+  if (valid) {
+    const double* v = src + S_V;
+    const double eps2 = src[S_EPS] * src[S_EPS];
+    const double ar2 = 2.0 * src[S_AREA];
+    const int off = nq * (nq - 1) / 2;
+    const double* gx = pr.gl_x + off;
+    const double* gw = pr.gl_w + off;
+    for (int g = lane; g < nq * nq; g += 64) {
+      const int i = g / nq, j = g - i * nq;
+      const double gi = gx[i];
+      const double x1 = gi, x2 = gx[j] * (1.0 - gi);
+      const double w = gw[i] * gw[j] * (1.0 - gi) * ar2;
+      const double l0 = 1.0 - x1 - x2;
+      const double y0 = l0 * v[0] + x1 * v[3] + x2 * v[6];
+      const double y1 = l0 * v[1] + x1 * v[4] + x2 * v[7];
+      const double y2 = l0 * v[2] + x1 * v[5] + x2 * v[8];
+      const double wn = w * shape_at<P>(k, l0, x1, x2);
+      const double d1 = ob[0] - y0, d2 = ob[1] - y1, d3 = ob[2] + y2;
+      const double r2 = sqrt(d1 * d1 + d2 * d2 + d3 * d3 + eps2);
+      q_kappa_point<NOUT>(d1, d2, d3, 1.0 / r2, 1.0 / (r2 - d3), wn, kap_s[wv], acc);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NOUT; ++t)
+    for (int off = 32; off > 0; off >>= 1) acc[t] += __shfl_down(acc[t], off, 64);
+  if (valid && lane == 0) {
+    const int64_t oo = o - pr.obs_lo;
+#pragma unroll
+    for (int t = 0; t < NOUT; ++t) pr.out[(oo * pr.n_cols + col) * NOUT + t] += acc[t];
+  }
+}
+
 // ------------------------------------------------------------------ fold --
 // Zero the fold-list counter (a fill kernel rather than a memset node, so the
 // launch sequence is safe to capture in a graph).
@@ -292,7 +384,15 @@ static int launch_all(const mhs_launch* L, const Params& pr, hipStream_t st) {
     if (check(hipGetLastError(), "k_pair launch")) return -1;
   }
   if (pr.kind == K_EIGEN) return 0;
-  if (stages & MHS_STAGE_Q) {
+  const int nout = pr.n_a * pr.n_b;
+  const bool contracted = pr.kind == K_INTERACTION && pr.sb && nout <= 2;
+  if ((stages & MHS_STAGE_Q) && contracted) {
+    const int64_t items = n_rows * pr.n_cols;
+    const dim3 qg((unsigned)((items + 3) / 4));
+    if (nout == 1) hipLaunchKernelGGL((k_qc<P, 1>), qg, block, 0, st, pr, items);
+    else hipLaunchKernelGGL((k_qc<P, 2>), qg, block, 0, st, pr, items);
+    if (check(hipGetLastError(), "k_qc launch")) return -1;
+  } else if (stages & MHS_STAGE_Q) {
     if (stress) hipLaunchKernelGGL((k_q<P, true>), grid, block, 0, st, pr);
     else hipLaunchKernelGGL((k_q<P, false>), grid, block, 0, st, pr);
     if (check(hipGetLastError(), "k_q launch")) return -1;

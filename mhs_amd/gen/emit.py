@@ -190,6 +190,60 @@ def emit_q(em: Emitter, name, records, n_ix):
     em.w()
 
 
+def emit_q_kappa(em: Emitter, records, n_ix):
+    """Contracted Q-family: per pair, kappa_k = sum over the records of
+    monomial k of C[rec] * G[slot] (G = the pair's output functional on the
+    81 DDG slots); per quadrature point, sum_k kappa_k * monomial_k."""
+    by_mono = defaultdict(list)
+    for r, rec in enumerate(records):
+        slot = slot_of(rec[:n_ix])
+        by_mono[tuple(rec[n_ix:n_ix + 5])].append((slot, r))
+    monos = sorted(by_mono)
+    base = em.q_ddg_base
+    ptr = [0]
+    slots, coefs = [], []
+    for m in monos:
+        for slot, r in by_mono[m]:
+            slots.append(slot)
+            coefs.append(base + r)
+        ptr.append(len(slots))
+    em.w(f"#define MHS_QM {len(monos)}")
+    em.w(f"#define MHS_QREC {len(slots)}")
+    em.w(f"__constant__ short mhs_qm_ptr[{len(ptr)}] = {{{', '.join(map(str, ptr))}}};")
+    em.w(f"__constant__ short mhs_qm_coef[{len(coefs)}] = {{{', '.join(map(str, coefs))}}};")
+    em.w(f"__constant__ unsigned char mhs_qm_slot[{len(slots)}] = {{{', '.join(map(str, slots))}}};")
+    em.w("// sum_k kap[o * MHS_QM + k] * monomial_k, per output o, into acc[o] (times w).")
+    em.w("template <int NOUT>")
+    em.w("__host__ __device__ __forceinline__ void q_kappa_point(double d1, double d2, double d3,")
+    em.w("    double ir, double iq, double w, const double* kap, double* acc)")
+    em.w("{")
+    emax = [max(m[i] for m in monos) for i in range(5)]
+    for nm, var, e in (("x", "d1", emax[0]), ("y", "d2", emax[1]), ("zz", "d3", emax[2]),
+                       ("r", "ir", emax[3]), ("q", "iq", emax[4])):
+        em.w(f"  const double {nm}1 = {var};")
+        for k in range(2, e + 1):
+            em.w(f"  const double {nm}{k} = {nm}{k - 1} * {var};")
+    dparts = sorted({m[:3] for m in monos})
+    rparts = sorted({m[3:] for m in monos})
+    for a, b, c in dparts:
+        f = [f"x{a}"] * (a > 0) + [f"y{b}"] * (b > 0) + [f"zz{c}"] * (c > 0)
+        em.w(f"  const double D{a}{b}{c} = {' * '.join(f) if f else '1.0'};")
+    for n, q in rparts:
+        f = [f"r{n}"] * (n > 0) + [f"q{q}"] * (q > 0)
+        em.w(f"  const double R{n}{q} = {' * '.join(f) if f else '1.0'};")
+    em.w("  double s[NOUT];")
+    em.w("#pragma unroll")
+    em.w("  for (int o = 0; o < NOUT; ++o) s[o] = 0.0;")
+    for k, (a, b, c, n, q) in enumerate(monos):
+        em.w(f"  {{ const double mm = D{a}{b}{c} * R{n}{q};")
+        em.w("#pragma unroll")
+        em.w(f"    for (int o = 0; o < NOUT; ++o) s[o] = fma(kap[o * {len(monos)} + {k}], mm, s[o]); }}")
+    em.w("#pragma unroll")
+    em.w("  for (int o = 0; o < NOUT; ++o) acc[o] = fma(w, s[o], acc[o]);")
+    em.w("}")
+    em.w()
+
+
 def main():
     t = json.loads(TABLE.read_text())
     em = Emitter()
@@ -204,6 +258,7 @@ def main():
     offs["r_dg"] = body.n_coef = len(body.coef_src)
     emit_r(body, "r_dg", t["r_dg"], 3, 27)
     offs["q_ddg"] = body.n_coef = len(body.coef_src)
+    body.q_ddg_base = offs["q_ddg"]
     emit_q(body, "q_ddg_point", t["q_ddg"], 4)
     offs["q_dg"] = body.n_coef = len(body.coef_src)
     emit_q(body, "q_dg_point", t["q_dg"], 3)
@@ -244,6 +299,10 @@ def main():
     em.w("}")
     em.w()
     em.lines += body.lines
+    kap = Emitter()
+    kap.q_ddg_base = offs["q_ddg"]
+    emit_q_kappa(kap, t["q_ddg"], 4)
+    em.lines += kap.lines
     OUT.write_text("\n".join(em.lines) + "\n")
     print(f"wrote {OUT} ({len(em.lines)} lines, {n_coef} coefficients)")
 
