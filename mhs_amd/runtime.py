@@ -269,7 +269,7 @@ def to_host(t, out=None):
     if out is None:
         return t.cpu().numpy()
     if out.dtype == np.float32:
-        out[...] = t.to(dtype=t.new_empty(0).float().dtype).cpu().numpy()
+        out[...] = t.float().cpu().numpy()
     else:
         out[...] = t.cpu().numpy()
     return out

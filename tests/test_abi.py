@@ -1,0 +1,69 @@
+"""The C-ABI library: it loads, exports every symbol include/mhs_amd.h
+declares, refuses bad descriptors before any GPU call, and its per-material
+coefficients equal the oracle's evaluation of the exact rational table."""
+import ctypes as C
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from mhs_amd import _lib
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("libmhs_amd.so not built (run __graft_entry__.build())")
+    return _lib.load()
+
+
+def test_exports_every_declared_symbol(lib):
+    header = (ROOT / "include" / "mhs_amd.h").read_text()
+    declared = set(re.findall(r"^\s*(?:int|const char\*)\s+(mhs_\w+)\(", header, re.M))
+    assert declared >= {"mhs_assemble", "mhs_gather", "mhs_strain", "mhs_histogram", "mhs_fill_material"}
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+def test_sizes(lib):
+    from mhs_amd import geometry
+    assert lib.mhs_src_stride() == geometry.SRC_STRIDE
+    assert lib.mhs_ncoef() == 2167
+    assert lib.mhs_abi_version() == 1
+
+
+def test_bad_descriptors_refused_on_host(lib):
+    from mhs_amd import _lib
+    assert lib.mhs_assemble(None, None) != 0
+    assert b"null launch" in lib.mhs_last_error()
+    L = _lib.Launch(abi_version=99)
+    assert lib.mhs_assemble(C.byref(L), None) != 0
+    assert b"ABI" in lib.mhs_last_error()
+    L = _lib.Launch(abi_version=1, order=5)
+    assert lib.mhs_assemble(C.byref(L), None) != 0
+    assert b"order" in lib.mhs_last_error()
+    assert lib.mhs_fill_material(0.0, 1.0, None) != 0
+
+
+def test_material_coefficients_match_oracle(lib):
+    """The library's coefficient vector against the oracle's own evaluation
+    of the records (image.py:95-104, 276-297)."""
+    from oracle import image as I
+    mu, lam = 30.0, 45.0
+    c = np.zeros(lib.mhs_ncoef())
+    assert lib.mhs_fill_material(mu, lam, c.ctypes.data) == 0
+    nu = lam / (2 * (lam + mu))
+    scale = 2 * (lam + mu) / (np.pi * mu * (lam + 2 * mu))
+    t = I.table()
+    want = []
+    for key in ("r_ddg", "r_dg"):
+        for rec in t[key]:
+            by_pz = {}
+            for pz, pnu, num, den in rec[-1]:
+                by_pz[pz] = by_pz.get(pz, 0.0) + (num / den) * nu ** pnu
+            want += [v * scale for _, v in sorted(by_pz.items())]
+    for key in ("q_ddg", "q_dg"):
+        for rec in t[key]:
+            want.append(sum((num / den) * nu ** pnu for _, pnu, num, den in rec[-1]) * scale)
+    assert np.allclose(c, np.array(want), rtol=1e-14, atol=0)
