@@ -1,16 +1,20 @@
 """Benchmark: interaction matrix, n = 10,000 strike/strike, one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode fp64]
-                    [--cpu-rows R] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode mixed|fp64]
+                    [--cpu-cols C] [--no-cpu]
 
 A step is one full assembly of the workload with every input already resident
-in HBM and the (10^4, 10^4, 1, 1) FP64 output left in HBM: the pair pass
-(direct + image R-family / folded mirror + eigenstress), the Q-family pass and
-the Boussinesq fold pass.  Prints ONE JSON line.  Per-pass kernel times come
-from HIP events on the launch stream; the roofline uses the host-computed
-branch histogram and the static FLOP model of mhs_amd/flops.py.  The CPU
-baseline is the FP64 oracle (a numpy restatement of the reference) timed on a
-seeded sample of receiver rows over this host's cores.
+in HBM and the (10^4, 10^4, 1, 1) output left in HBM: the pair pass (direct +
+image R-family / folded mirror + eigenstress), the Q-family pass and the
+Boussinesq fold pass.  ``value`` is pair-evals/s of the requested mode (mixed
+by default: far-field quadrature points in FP32, FP64 accumulation); the other
+mode is timed beside it and the mixed result's max error relative to the FP64
+result is measured on all 10^8 entries.  Per-pass kernel times come from HIP
+events recorded on the launch stream; the roofline of the dominant kernel uses
+the kernels' own branch histogram and the static FLOP model of
+mhs_amd/flops.py.  The CPU baseline is the FP64 oracle (a numpy restatement
+of the reference) on every receiver against a seeded sample of source
+columns, the columns split over this host's cores.  Prints ONE JSON line.
 """
 from __future__ import annotations
 
@@ -26,89 +30,88 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "pair-evals/s, interaction n=10k, mixed vs FP64 @1e-6 rel; served p50/p99"
-FP64_PEAK_TFLOPS = 78.6
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}     # MI355X vector FP64, FP32 (published)
+CPU_SEED = 20261203
 
 
-def _cpu_rows(args):
-    rows, = args
+def _cpu_block(cols):
     from mhs_amd import workloads
     from oracle import api as oapi
     tris, eps = workloads.c3_interaction()
-    out = []
-    for r in rows:
-        out.append(oapi.interaction_matrix(tris, 30.0, 30.0, eps, receiver="strike", source="strike",
-                                           rows=(int(r), int(r) + 1)))
-    return len(rows)
+    K = oapi.interaction_matrix(tris[cols], 30.0, 30.0, eps, receiver="strike", source="strike",
+                                obs_tris=tris)
+    return int(K.shape[0] * K.shape[1])
 
 
-def cpu_baseline(n_rows: int, cores: int) -> dict:
-    """The FP64 oracle on ``n_rows`` seeded receiver rows of the workload, one
-    process per core (BLAS pinned to one thread), pair-evals/s."""
+def cpu_baseline(n_cols: int, cores: int) -> dict:
+    """The FP64 oracle on all 10^4 receivers x ``n_cols`` seeded source
+    columns (the reference's own loop runs per source, vectorised over the
+    receivers), one column per task on a pool of one spawned process per core
+    (BLAS pinned to one thread), the surface-reaching (folded, costliest)
+    columns first; pair-evals/s over the wall time.  The sample is
+    stratified: surface-reaching sources (shallowest vertex above 3 eps
+    depth) and the rest are drawn in proportion to their counts."""
     import multiprocessing as mp
-    rng = np.random.default_rng(20261203)
-    rows = np.sort(rng.choice(10_000, size=n_rows, replace=False))
-    chunks = [rows[i::cores] for i in range(cores) if len(rows[i::cores])]
+    from mhs_amd import workloads
+    rng = np.random.default_rng(CPU_SEED)
+    tris, eps = workloads.c3_interaction()
+    shallow = tris[:, :, 2].max(axis=1) > -3.0 * eps
+    strad, rest = np.flatnonzero(shallow), np.flatnonzero(~shallow)
+    n_sh = int(round(n_cols * len(strad) / len(tris)))
+    cols = np.concatenate([rng.choice(strad, size=n_sh, replace=False),
+                           rng.choice(rest, size=n_cols - n_sh, replace=False)])
+    blocks = [np.array([c]) for c in cols]
     env = {"OMP_NUM_THREADS": "1", "OPENBLAS_NUM_THREADS": "1", "MKL_NUM_THREADS": "1"}
     old = {k: os.environ.get(k) for k in env}
     os.environ.update(env)
     try:
         ctx = mp.get_context("spawn")
-        t0 = time.perf_counter()
-        with ctx.Pool(len(chunks)) as pool:
-            done = sum(pool.map(_cpu_rows, [(c,) for c in chunks]))
-        dt = time.perf_counter() - t0
+        with ctx.Pool(min(cores, len(blocks))) as pool:
+            t0 = time.perf_counter()
+            pairs = sum(pool.imap_unordered(_cpu_block, blocks, chunksize=1))
+            dt = time.perf_counter() - t0
     finally:
         for k, v in old.items():
             if v is None:
                 os.environ.pop(k, None)
             else:
                 os.environ[k] = v
-    pairs = done * 10_000
-    return {"value": pairs / dt, "unit": "pair-evals/s", "cores": len(chunks), "kind": "port",
-            "sample": f"{done} seeded receiver rows x 10,000 sources of the same workload "
-                      f"({pairs:.2e} pairs incl. their fold pairs) in {dt:.1f} s wall"}
+    used = min(cores, len(blocks))
+    return {"value": pairs / dt, "unit": "pair-evals/s", "cores": used, "kind": "port",
+            "sample": f"all 10,000 receivers x {n_cols} seeded source columns (seed {CPU_SEED}; {n_sh} of them "
+                      f"surface-reaching, as in the whole mesh) of the same workload, FP64 oracle, "
+                      f"{pairs:.3e} pairs (their fold pairs included) in "
+                      f"{dt:.1f} s wall on {used} processes"}
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", default="fp64", choices=["fp64"])
-    ap.add_argument("--cpu-rows", type=int, default=32)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the assembly shards by "
-              f"receiver rows with no exchange, and no multi-GPU figure is produced here.",
-              file=sys.stderr)
-        return 2
-
-    import torch
-    from mhs_amd import _lib, flops, runtime, tdcs, workloads
+def _job(mixed):
+    from mhs_amd import runtime, tdcs, workloads
     from mhs_amd.api import collocation_points
-
     tris, eps = workloads.c3_interaction()
     n = tris.shape[0]
-    obs = collocation_points(tris, 0)
     fr = tdcs.slip_frame(tris)
-    job = runtime.Job(kind="interaction", order=0, obs=obs, tris=tris, eps=np.full(n, eps),
-                      mu=30.0, lam=30.0, fold=True, subtract=True,
-                      nrm=np.ascontiguousarray(fr[:, 2, :]), rb=np.ascontiguousarray(fr[:, 0:1, :]),
-                      sb=np.ascontiguousarray(fr[:, 0:1, :]))
-    prep = runtime.Prepared(job)
+    return runtime.Job(kind="interaction", order=0, obs=collocation_points(tris, 0), tris=tris,
+                       eps=np.full(n, eps), mu=30.0, lam=30.0, fold=True, subtract=True,
+                       nrm=np.ascontiguousarray(fr[:, 2, :]), rb=np.ascontiguousarray(fr[:, 0:1, :]),
+                       sb=np.ascontiguousarray(fr[:, 0:1, :]), mixed=mixed)
+
+
+def _time_mode(mixed, steps, warmup):
+    """Steps of one mode: wall ms per step and per-pass HIP-event ms."""
+    import torch
+    from mhs_amd import _lib, runtime
+    prep = runtime.Prepared(_job(mixed))
     st = prep.stream
     with torch.cuda.stream(st):
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             prep.launch()
         torch.cuda.synchronize()
         prep.check_fold()
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             prep.launch()
         torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / args.steps
-        # per-pass kernel time with HIP events on the launch stream
+        dt = (time.perf_counter() - t0) / steps
         stage_ms = {}
         for name, mask in (("pair", _lib.STAGE_PAIR), ("q", _lib.STAGE_Q), ("fold", _lib.STAGE_FOLD)):
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -117,12 +120,49 @@ def main() -> int:
             e1.record(st)
             e1.synchronize()
             stage_ms[name] = e0.elapsed_time(e1)
+        prep.launch()                    # leave a whole result for the error check
+        torch.cuda.synchronize()
+    return prep, dt, stage_ms
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--mode", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--cpu-cols", type=int, default=48)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the assembly shards by "
+              f"receiver rows with no exchange, and no multi-GPU figure is produced here.",
+              file=sys.stderr)
+        return 2
+
+    from mhs_amd import flops
+    mixed = args.mode == "mixed"
+    prep_o, dt_o, stage_o = _time_mode(not mixed, args.steps, args.warmup)
+    ref = prep_o.final.clone()
+    hist = prep_o.histogram()
+    del prep_o
+    prep, dt, stage_ms = _time_mode(mixed, args.steps, args.warmup)
+    a, b = (prep.final, ref) if mixed else (ref, prep.final)
+    relerr = float(((a - b).abs().max() / b.abs().max()).item())
+    del prep, ref
+
+    n = 10_000
     pairs = n * n
     value = pairs / dt
-    hist = prep.histogram()
     fl = flops.flops(hist)
-    achieved_q = fl["q"] / (stage_ms["q"] * 1e-3) / 1e12
-    achieved_all = (fl["pair"] + fl["q"]) / ((stage_ms["pair"] + stage_ms["q"]) * 1e-3) / 1e12
+    q_s = stage_ms["q"] * 1e-3
+    if mixed:
+        f32 = fl["q_fp32_in_mixed"]
+        f64 = fl["q"] - f32
+    else:
+        f32, f64 = 0.0, fl["q"]
+    achieved = (f32 + f64) / q_s / 1e12
+    peak = (f32 + f64) / (f32 / PEAK_TFLOPS["f32"] + f64 / PEAK_TFLOPS["f64"])
     line = {
         "metric": METRIC,
         "value": value,
@@ -134,23 +174,26 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if mixed else "f64",
         "data": "synthetic",
-        "config": {"workload": "C3: interaction_matrix receiver=source='strike', n=10,000 triangles, "
-                               "vertical strike-slip fault 100 x 15 km reaching z=0 (fold active), "
-                               "jitter 5%, eps=0.1, Material(30, 30), order 0, FP64",
+        "config": {"workload": "interaction_matrix receiver=source='strike', n=10,000 triangles, vertical "
+                               "strike-slip fault 100 x 15 km reaching z=0 (fold active), seeded 5% vertex "
+                               "jitter, eps=0.1, Material(30, 30), order 0",
                    "n": n, "pairs": pairs, "mode": args.mode},
-        "roofline": {"bound": "valu", "kernel": "k_q (image Q-family, FP64)",
-                     "achieved": achieved_q, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved_q / FP64_PEAK_TFLOPS, "traffic": None,
-                     "all_passes_achieved": achieved_all},
+        "roofline": {"bound": "valu", "kernel": f"k_qc (image Q-family, contracted, {args.mode})",
+                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": None},
         "stage_ms": stage_ms,
+        "other_mode": {"mode": "fp64" if mixed else "mixed", "ms_per_step": dt_o * 1e3,
+                       "value": pairs / dt_o, "stage_ms": stage_o},
+        "mixed_over_fp64": (dt_o / dt) if mixed else (dt / dt_o),
+        "mixed_relmax_vs_fp64": relerr,
         "histogram": hist,
         "flop": fl,
     }
     if not args.no_cpu:
         cores = min(os.cpu_count() or 1, 16)
-        line["cpu_baseline"] = cpu_baseline(args.cpu_rows, cores)
+        line["cpu_baseline"] = cpu_baseline(args.cpu_cols, cores)
         line["vs_cpu"] = value / line["cpu_baseline"]["value"]
     print(json.dumps(line), flush=True)
     return 0

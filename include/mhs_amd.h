@@ -34,8 +34,10 @@ extern "C" {
 enum { MHS_DISP = 0, MHS_STRESS = 1, MHS_EIGEN = 2, MHS_TRACTION = 3, MHS_INTERACTION = 4 };
 
 /* Flags: fold surface-reaching sources (fold=True), subtract the eigenstress
- * (subtract_eigenstress=True), also write the eigenstress to out2 (parts=True). */
-enum { MHS_FOLD = 1, MHS_SUBTRACT = 2, MHS_PARTS = 4 };
+ * (subtract_eigenstress=True), also write the eigenstress to out2 (parts=True),
+ * mixed precision (MHS_MIXED: far-field quadrature points in FP32 with FP64
+ * accumulation; near field, fold and eigenstress stay FP64). */
+enum { MHS_FOLD = 1, MHS_SUBTRACT = 2, MHS_PARTS = 4, MHS_MIXED = 8 };
 
 /* Passes of one assembly: the pair pass writes every entry (direct + image
  * R-family or folded mirror + eigenstress), the Q-family and the Boussinesq

@@ -14,12 +14,14 @@ from . import chunking, defaults, tdcs
 from .api import (collocation_points, disp_matrix, eigenstress_matrix, elastic_strain_matrix,
                   interaction_matrix, stress_matrix, total_stress_matrix, traction_matrix)
 from .materials import Material
+from .runtime import get_precision, precision, set_precision
 from .shared import SharedNodes, shared_nodes
 
 __all__ = [
     "Material", "disp_matrix", "stress_matrix", "traction_matrix", "interaction_matrix",
     "total_stress_matrix", "eigenstress_matrix", "elastic_strain_matrix", "collocation_points",
-    "shared_nodes", "SharedNodes", "chunking", "defaults", "tdcs",
+    "shared_nodes", "SharedNodes", "chunking", "defaults", "tdcs", "precision", "set_precision",
+    "get_precision",
 ]
 
 __version__ = "0.1.0"

@@ -13,7 +13,7 @@ LIB_PATH = pathlib.Path(__file__).resolve().parent / "libmhs_amd.so"
 ABI_VERSION = 1
 
 KIND = {"disp": 0, "stress": 1, "eigen": 2, "traction": 3, "interaction": 4}
-F_FOLD, F_SUBTRACT, F_PARTS = 1, 2, 4
+F_FOLD, F_SUBTRACT, F_PARTS, F_MIXED = 1, 2, 4, 8
 STAGE_PAIR, STAGE_Q, STAGE_FOLD = 1, 2, 4
 
 

@@ -114,7 +114,8 @@ def _finish(job, buf):
 
 def _job(kind, obs, tri, mu, lam, eps_arr, p, ids, sn, fold, **kw):
     return runtime.Job(kind=kind, order=p, obs=obs, tris=tri, eps=eps_arr, mu=mu, lam=lam,
-                       fold=bool(fold), ids=ids, n_nodes=(sn.n if sn is not None else 0), **kw)
+                       fold=bool(fold), ids=ids, n_nodes=(sn.n if sn is not None else 0),
+                       mixed=runtime.get_precision() == "mixed", **kw)
 
 
 def disp_matrix(obs_pts, tris, material, eps, *, order=0, layout="discontinuous", shared=None,

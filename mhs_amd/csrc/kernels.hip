@@ -18,6 +18,13 @@
 #include "fold.hpp"
 #include "../../include/mhs_amd.h"
 
+#ifndef MHS_QC_NPT
+#define MHS_QC_NPT 4   // quadrature points per lane in the contracted Q pass
+#endif
+#ifndef MHS_QC_WAVES
+#define MHS_QC_WAVES 2 // occupancy target of the contracted Q pass (waves per SIMD)
+#endif
+
 static thread_local char g_err[512];
 
 static int fail(const char* msg) {
@@ -33,12 +40,18 @@ static int check(hipError_t e, const char* what) {
   return 0;
 }
 
+// Blocks of 256 for a grid-stride loop over n items, capped so the dispatch's
+// work-item count stays far inside 32 bits.
+static unsigned grid_1d(int64_t n) {
+  const int64_t b = (n + 255) / 256;
+  return (unsigned)(b < 1 ? 1 : (b > (1 << 20) ? (1 << 20) : b));
+}
+
 // ------------------------------------------------------------------ pair --
 template <int P, int MODE>   // MODE 0: stress family, 1: displacement, 2: eigenstress only
 __global__ void __launch_bounds__(256) k_pair(Params pr) {
-  const int64_t nblk = (pr.n_cols + 255) / 256;
-  const int64_t o = pr.obs_lo + blockIdx.x / nblk;
-  const int64_t col = (blockIdx.x % nblk) * 256 + threadIdx.x;
+  const int64_t o = pr.obs_lo + blockIdx.y;
+  const int64_t col = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (col >= pr.n_cols) return;
   pair_item<P, MODE>(pr, o, col);
 }
@@ -46,9 +59,8 @@ __global__ void __launch_bounds__(256) k_pair(Params pr) {
 // --------------------------------------------------------------- Q-family --
 template <int P, bool STRESS>
 __global__ void __launch_bounds__(256) k_q(Params pr) {
-  const int64_t nblk = (pr.n_cols + 255) / 256;
-  const int64_t o = pr.obs_lo + blockIdx.x / nblk;
-  const int64_t col = (blockIdx.x % nblk) * 256 + threadIdx.x;
+  const int64_t o = pr.obs_lo + blockIdx.y;
+  const int64_t col = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (col >= pr.n_cols) return;
   q_item<P, STRESS>(pr, o, col);
 }
@@ -58,23 +70,30 @@ __global__ void __launch_bounds__(256) k_q(Params pr) {
 // Interaction with one or two output components: the Q-family of a pair
 // collapses to sum_k kappa_k * monomial_k, kappa = the records contracted with
 // the pair's output functional G[slot] = E_a[i][p] F_b[j][m] (receiver and
-// source halves of the traction-operator / Hooke / frame chain).  One wave per
-// pair: lanes share out the quadrature points and kappa lives in LDS.
-template <int P, int NOUT>
-__global__ void __launch_bounds__(256) k_qc(Params pr, int64_t n_items) {
+// source halves of the traction-operator / Hooke / frame chain).
+//
+// LPP = 64 / NPT lanes per pair, NPT quadrature points per lane per pass, so
+// one pass covers 64 points (the budget law's floor of 8 x 8).  kappa lives in
+// LDS and every kappa read feeds NPT FMAs.  T is the point arithmetic (double,
+// or float in the mixed mode); the point geometry, the weights and the
+// per-lane and cross-lane sums stay in double.
+template <typename T, int P, int NOUT, int NPT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MHS_QC_WAVES)))
+k_qc(Params pr) {
   constexpr int K = (P + 1) * (P + 2) / 2;
-  __shared__ double kap_s[4][NOUT * MHS_QM];
-  __shared__ double g_s[4][NOUT * 81];
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t item = (int64_t)blockIdx.x * 4 + wv;
-  bool valid = item < n_items;
-  int64_t o = 0, col = 0, s = 0;
+  constexpr int LPP = 64 / NPT;
+  constexpr int PPB = 256 / LPP;
+  __shared__ T kap_s[PPB][NOUT * MHS_QM];
+  __shared__ double g_s[PPB][NOUT * 81];
+  const int grp = threadIdx.x / LPP, li = threadIdx.x % LPP;
+  const int64_t o = pr.obs_lo + blockIdx.y;
+  const int64_t col = (int64_t)blockIdx.x * PPB + grp;
+  bool valid = col < pr.n_cols;
+  int64_t s = 0;
   int k = 0, nq = 0;
   const double* src = pr.src;
   double ob[3] = {0.0, 0.0, 0.0};
   if (valid) {
-    o = pr.obs_lo + item / pr.n_cols;
-    col = item % pr.n_cols;
     s = col / K;
     k = (int)(col - s * K);
     src = pr.src + s * MHS_SRC_STRIDE;
@@ -82,11 +101,10 @@ __global__ void __launch_bounds__(256) k_qc(Params pr, int64_t n_items) {
     valid = !in_fold_band(src, ob, pr.flags);
     if (valid) nq = q_order<P>(src, ob);
   }
-  // output functional G[out][slot] = E_a[i][p] * F_b[j][m]
   if (valid) {
     const double* nr = pr.nrm + 3 * o;
     const double* n = src + S_N;
-    for (int t = lane; t < NOUT * 81; t += 64) {
+    for (int t = li; t < NOUT * 81; t += LPP) {
       const int out = t / 81, slot = t % 81;
       const int a = out / pr.n_b, b = out % pr.n_b;
       const double* rb = pr.rb + ((int64_t)o * pr.n_a + a) * 3;
@@ -96,17 +114,17 @@ __global__ void __launch_bounds__(256) k_qc(Params pr, int64_t n_items) {
       const double sn = sb[0] * n[0] + sb[1] * n[1] + sb[2] * n[2];
       const double E = pr.mu * (rb[i] * nr[p] + rb[p] * nr[i]) + (i == p ? pr.lam * rn : 0.0);
       const double F = pr.mu * (sb[j] * n[m] + n[j] * sb[m]) + (j == m ? pr.lam * sn : 0.0);
-      g_s[wv][t] = E * F;
+      g_s[grp][t] = E * F;
     }
   }
   __syncthreads();
   if (valid) {
-    for (int t = lane; t < NOUT * MHS_QM; t += 64) {
+    for (int t = li; t < NOUT * MHS_QM; t += LPP) {
       const int out = t / MHS_QM, km = t % MHS_QM;
       double acc = 0.0;
       for (int r = mhs_qm_ptr[km]; r < mhs_qm_ptr[km + 1]; ++r)
-        acc = fma(pr.coef[mhs_qm_coef[r]], g_s[wv][out * 81 + mhs_qm_slot[r]], acc);
-      kap_s[wv][t] = acc;
+        acc = fma(pr.coef[mhs_qm_coef[r]], g_s[grp][out * 81 + mhs_qm_slot[r]], acc);
+      kap_s[grp][t] = (T)acc;
     }
   }
   __syncthreads();
@@ -120,26 +138,45 @@ __global__ void __launch_bounds__(256) k_qc(Params pr, int64_t n_items) {
     const int off = nq * (nq - 1) / 2;
     const double* gx = pr.gl_x + off;
     const double* gw = pr.gl_w + off;
-    for (int g = lane; g < nq * nq; g += 64) {
-      const int i = g / nq, j = g - i * nq;
-      const double gi = gx[i];
-      const double x1 = gi, x2 = gx[j] * (1.0 - gi);
-      const double w = gw[i] * gw[j] * (1.0 - gi) * ar2;
-      const double l0 = 1.0 - x1 - x2;
-      const double y0 = l0 * v[0] + x1 * v[3] + x2 * v[6];
-      const double y1 = l0 * v[1] + x1 * v[4] + x2 * v[7];
-      const double y2 = l0 * v[2] + x1 * v[5] + x2 * v[8];
-      const double wn = w * shape_at<P>(k, l0, x1, x2);
-      const double d1 = ob[0] - y0, d2 = ob[1] - y1, d3 = ob[2] + y2;
-      const double r2 = sqrt(d1 * d1 + d2 * d2 + d3 * d3 + eps2);
-      q_kappa_point<NOUT>(d1, d2, d3, 1.0 / r2, 1.0 / (r2 - d3), wn, kap_s[wv], acc);
+    const int npts = nq * nq;
+    for (int base = 0; base < npts; base += 64) {
+      // compiler-only fence: keeps the kappa reads inside the pass instead of
+      // hoisting all of them (2 * MHS_QM registers) out of the loop
+      __asm__ volatile("" ::: "memory");
+      T x[NPT], y[NPT], z[NPT], ir[NPT], iq[NPT], sv[NOUT * NPT];
+      double wn[NPT];
+#pragma unroll
+      for (int u = 0; u < NPT; ++u) {
+        const int g0 = base + u * LPP + li;
+        const bool in = g0 < npts;
+        const int g = in ? g0 : 0;
+        const int i = g / nq, j = g - i * nq;
+        const double gi = gx[i];
+        const double x1 = gi, x2 = gx[j] * (1.0 - gi);
+        const double w = in ? gw[i] * gw[j] * (1.0 - gi) * ar2 : 0.0;
+        const double l0 = 1.0 - x1 - x2;
+        const double y0 = l0 * v[0] + x1 * v[3] + x2 * v[6];
+        const double y1 = l0 * v[1] + x1 * v[4] + x2 * v[7];
+        const double y2 = l0 * v[2] + x1 * v[5] + x2 * v[8];
+        wn[u] = w * shape_at<P>(k, l0, x1, x2);
+        const T d1 = (T)(ob[0] - y0), d2 = (T)(ob[1] - y1), d3 = (T)(ob[2] + y2);
+        const T R = sqrt(d1 * d1 + d2 * d2 + d3 * d3 + (T)eps2);
+        x[u] = d1; y[u] = d2; z[u] = d3;
+        ir[u] = (T)1 / R;
+        iq[u] = (T)1 / (R - d3);
+      }
+      q_blocked<T, NPT, NOUT>(kap_s[grp], x, y, z, ir, iq, sv);
+#pragma unroll
+      for (int t = 0; t < NOUT; ++t)
+#pragma unroll
+        for (int u = 0; u < NPT; ++u) acc[t] = fma(wn[u], (double)sv[t * NPT + u], acc[t]);
     }
   }
 #pragma unroll
   for (int t = 0; t < NOUT; ++t)
-    for (int off = 32; off > 0; off >>= 1) acc[t] += __shfl_down(acc[t], off, 64);
-  if (valid && lane == 0) {
-    const int64_t oo = o - pr.obs_lo;
+    for (int off = LPP / 2; off > 0; off >>= 1) acc[t] += __shfl_xor(acc[t], off, LPP);
+  if (valid && li == 0) {
+    const int64_t oo = o - pr.out_lo;
 #pragma unroll
     for (int t = 0; t < NOUT; ++t) pr.out[(oo * pr.n_cols + col) * NOUT + t] += acc[t];
   }
@@ -157,17 +194,18 @@ __global__ void k_zero_i32(int* p, int n) {
 __global__ void k_fold_list(Params pr, const int64_t* __restrict__ strad, int64_t n_strad,
                             int64_t n_rows, int64_t* __restrict__ list, int* __restrict__ count,
                             int64_t cap) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_rows * n_strad) return;
-  const int64_t o = pr.obs_lo + t / n_strad;
-  const int64_t s = strad[t % n_strad];
-  const double* src = pr.src + s * MHS_SRC_STRIDE;
-  const double ob[3] = {pr.obs[3 * o], pr.obs[3 * o + 1], pr.obs[3 * o + 2]};
-  if (!in_fold_band(src, ob, pr.flags)) return;
-  const int slot = atomicAdd(count, 1);
-  if (slot < cap) {
-    list[2 * slot] = o;
-    list[2 * slot + 1] = s;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n_rows * n_strad; t += stride) {
+    const int64_t o = pr.obs_lo + t / n_strad;
+    const int64_t s = strad[t % n_strad];
+    const double* src = pr.src + s * MHS_SRC_STRIDE;
+    const double ob[3] = {pr.obs[3 * o], pr.obs[3 * o + 1], pr.obs[3 * o + 2]};
+    if (!in_fold_band(src, ob, pr.flags)) continue;
+    const int slot = atomicAdd(count, 1);
+    if (slot < cap) {
+      list[2 * slot] = o;
+      list[2 * slot + 1] = s;
+    }
   }
 }
 
@@ -286,9 +324,8 @@ __global__ void k_hist(Params pr, int order, unsigned long long* __restrict__ co
   if (threadIdx.x < 13) acc[threadIdx.x] = 0ull;
   __syncthreads();
   const int K = (order + 1) * (order + 2) / 2;
-  const int64_t nblk = (pr.n_cols + 255) / 256;
-  const int64_t o = pr.obs_lo + blockIdx.x / nblk;
-  const int64_t col = (blockIdx.x % nblk) * 256 + threadIdx.x;
+  const int64_t o = pr.obs_lo + blockIdx.y;
+  const int64_t col = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (col < pr.n_cols) {
     const int64_t s = col / K;
     const double* src = pr.src + s * MHS_SRC_STRIDE;
@@ -323,21 +360,22 @@ __global__ void k_hist(Params pr, int order, unsigned long long* __restrict__ co
 __global__ void k_gather(const double* __restrict__ scr, double* __restrict__ out, int64_t n_lead,
                          int64_t n_elem_cols, int64_t n_nodes, int tail, const int64_t* __restrict__ ptr,
                          const int64_t* __restrict__ idx, const double* __restrict__ sb, int n_a, int n_b) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   if (sb == nullptr) {
     // layout (n_lead, n_cols, tail)
     const int64_t total = n_lead * n_nodes * tail;
-    if (t >= total) return;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += stride) {
     const int64_t e = t % tail;
     const int64_t g = (t / tail) % n_nodes;
     const int64_t L = t / (tail * n_nodes);
     double acc = 0.0;
     for (int64_t q = ptr[g]; q < ptr[g + 1]; ++q) acc += scr[(L * n_elem_cols + idx[q]) * tail + e];
     out[t] = acc;
+    }
   } else {
     // interaction: scratch (n_obs, n_elem_cols, n_a, 3) -> out (n_obs, n_nodes, n_a, n_b)
     const int64_t total = n_lead * n_nodes * n_a * n_b;
-    if (t >= total) return;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += stride) {
     const int b = (int)(t % n_b);
     const int a = (int)((t / n_b) % n_a);
     const int64_t g = (t / ((int64_t)n_a * n_b)) % n_nodes;
@@ -349,59 +387,77 @@ __global__ void k_gather(const double* __restrict__ scr, double* __restrict__ ou
       acc += v[0] * bv[0] + v[1] * bv[1] + v[2] * bv[2];
     }
     out[t] = acc;
+    }
   }
 }
 
 // Elastic strain from stress (matrices.py:572-579): tr(e) = tr(sig)/(3 lam + 2 mu),
 // e = (sig - lam tr(e) I) / (2 mu), in place on (n_obs, 3, 3, n_cols, 3).
 __global__ void k_strain(double* __restrict__ buf, int64_t n_obs, int64_t n_cols, double mu, double lam) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_obs * n_cols * 3) return;
-  const int64_t j = t % 3;
-  const int64_t c = (t / 3) % n_cols;
-  const int64_t o = t / (3 * n_cols);
-  auto at = [&](int a, int b) -> double& { return buf[((o * 9 + a * 3 + b) * n_cols + c) * 3 + j]; };
-  const double tre = (at(0, 0) + at(1, 1) + at(2, 2)) / (3.0 * lam + 2.0 * mu);
-  for (int a = 0; a < 3; ++a) at(a, a) -= lam * tre;
-  for (int a = 0; a < 3; ++a)
-    for (int b = 0; b < 3; ++b) at(a, b) /= (2.0 * mu);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n_obs * n_cols * 3; t += stride) {
+    const int64_t j = t % 3;
+    const int64_t c = (t / 3) % n_cols;
+    const int64_t o = t / (3 * n_cols);
+    auto at = [&](int a, int b) -> double& { return buf[((o * 9 + a * 3 + b) * n_cols + c) * 3 + j]; };
+    const double tre = (at(0, 0) + at(1, 1) + at(2, 2)) / (3.0 * lam + 2.0 * mu);
+    for (int a = 0; a < 3; ++a) at(a, a) -= lam * tre;
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) at(a, b) /= (2.0 * mu);
+  }
 }
+
+// Row-chunked 2-D launches: blockIdx.y is the observer row within a chunk of
+// at most MHS_ROWS_PER_LAUNCH rows and blockIdx.x the column tile, so neither
+// grid dimension nor the work-item count of one dispatch (32-bit in the AQL
+// packet) can overflow however large the matrix.
+#define MHS_ROWS_PER_LAUNCH 32768
 
 template <int P>
 static int launch_all(const mhs_launch* L, const Params& pr, hipStream_t st) {
   const int64_t n_rows = L->obs_hi - L->obs_lo;
+  if (n_rows <= 0 || pr.n_cols <= 0) return 0;
   const int64_t nblk = (pr.n_cols + 255) / 256;
-  const int64_t blocks = n_rows * nblk;
-  if (blocks <= 0) return 0;
-  if (blocks > 0x7fffffffLL) return fail("mhs_assemble: launch too large; split the observer rows");
-  const dim3 grid((unsigned)blocks), block(256);
+  if (nblk * 256 > 0xffffffffLL) return fail("mhs_assemble: too many columns for one launch");
+  const dim3 block(256);
   const bool stress = pr.kind != K_DISP;
   const int stages = L->stages ? L->stages : (MHS_STAGE_PAIR | MHS_STAGE_Q | MHS_STAGE_FOLD);
-  if (stages & MHS_STAGE_PAIR) {
-    if (pr.kind == K_DISP) hipLaunchKernelGGL((k_pair<P, 1>), grid, block, 0, st, pr);
-    else if (pr.kind == K_EIGEN) hipLaunchKernelGGL((k_pair<P, 2>), grid, block, 0, st, pr);
-    else hipLaunchKernelGGL((k_pair<P, 0>), grid, block, 0, st, pr);
-    if (check(hipGetLastError(), "k_pair launch")) return -1;
-  }
-  if (pr.kind == K_EIGEN) return 0;
   const int nout = pr.n_a * pr.n_b;
   const bool contracted = pr.kind == K_INTERACTION && pr.sb && nout <= 2;
-  if ((stages & MHS_STAGE_Q) && contracted) {
-    const int64_t items = n_rows * pr.n_cols;
-    const dim3 qg((unsigned)((items + 3) / 4));
-    if (nout == 1) hipLaunchKernelGGL((k_qc<P, 1>), qg, block, 0, st, pr, items);
-    else hipLaunchKernelGGL((k_qc<P, 2>), qg, block, 0, st, pr, items);
-    if (check(hipGetLastError(), "k_qc launch")) return -1;
-  } else if (stages & MHS_STAGE_Q) {
-    if (stress) hipLaunchKernelGGL((k_q<P, true>), grid, block, 0, st, pr);
-    else hipLaunchKernelGGL((k_q<P, false>), grid, block, 0, st, pr);
-    if (check(hipGetLastError(), "k_q launch")) return -1;
+  const bool mixed = pr.flags & F_MIXED;
+  constexpr int NPT = MHS_QC_NPT;
+  constexpr int PPB = 256 / (64 / NPT);
+  for (int64_t lo = L->obs_lo; lo < L->obs_hi; lo += MHS_ROWS_PER_LAUNCH) {
+    const int64_t rows = (L->obs_hi - lo) < MHS_ROWS_PER_LAUNCH ? (L->obs_hi - lo) : MHS_ROWS_PER_LAUNCH;
+    Params pc = pr;
+    pc.obs_lo = lo;
+    const dim3 grid((unsigned)nblk, (unsigned)rows);
+    if (stages & MHS_STAGE_PAIR) {
+      if (pr.kind == K_DISP) hipLaunchKernelGGL((k_pair<P, 1>), grid, block, 0, st, pc);
+      else if (pr.kind == K_EIGEN) hipLaunchKernelGGL((k_pair<P, 2>), grid, block, 0, st, pc);
+      else hipLaunchKernelGGL((k_pair<P, 0>), grid, block, 0, st, pc);
+      if (check(hipGetLastError(), "k_pair launch")) return -1;
+    }
+    if (pr.kind == K_EIGEN || !(stages & MHS_STAGE_Q)) continue;
+    if (contracted) {
+      const dim3 qg((unsigned)((pr.n_cols + PPB - 1) / PPB), (unsigned)rows);
+      if (nout == 1 && mixed) hipLaunchKernelGGL((k_qc<float, P, 1, NPT>), qg, block, 0, st, pc);
+      else if (nout == 1) hipLaunchKernelGGL((k_qc<double, P, 1, NPT>), qg, block, 0, st, pc);
+      else if (mixed) hipLaunchKernelGGL((k_qc<float, P, 2, NPT>), qg, block, 0, st, pc);
+      else hipLaunchKernelGGL((k_qc<double, P, 2, NPT>), qg, block, 0, st, pc);
+      if (check(hipGetLastError(), "k_qc launch")) return -1;
+    } else {
+      if (stress) hipLaunchKernelGGL((k_q<P, true>), grid, block, 0, st, pc);
+      else hipLaunchKernelGGL((k_q<P, false>), grid, block, 0, st, pc);
+      if (check(hipGetLastError(), "k_q launch")) return -1;
+    }
   }
+  if (pr.kind == K_EIGEN) return 0;
   if ((stages & MHS_STAGE_FOLD) && (pr.flags & F_FOLD) && L->n_strad > 0) {
     if (!L->fold_list || !L->fold_count || !L->fold_err) return fail("mhs_assemble: fold buffers missing");
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, L->fold_count, 2);
     const int64_t nt = n_rows * L->n_strad;
-    hipLaunchKernelGGL(k_fold_list, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, pr, L->strad,
+    hipLaunchKernelGGL(k_fold_list, dim3(grid_1d(nt)), dim3(256), 0, st, pr, L->strad,
                        L->n_strad, n_rows, L->fold_list, L->fold_count, L->fold_cap);
     if (check(hipGetLastError(), "k_fold_list launch")) return -1;
     const dim3 fg(2048);
@@ -453,6 +509,7 @@ int mhs_assemble(const mhs_launch* L, void* stream) {
   pr.obs = L->obs; pr.nrm = L->nrm; pr.rb = L->rb; pr.sb = L->sb; pr.src = L->src;
   pr.coef = L->coef; pr.gl_x = L->gl_x; pr.gl_w = L->gl_w; pr.out = L->out; pr.out2 = L->out2;
   pr.n_obs = L->n_obs; pr.n_src = L->n_src; pr.n_cols = L->n_cols; pr.obs_lo = L->obs_lo;
+  pr.out_lo = L->obs_lo;
   pr.kind = L->kind; pr.flags = L->flags; pr.n_a = L->n_a; pr.n_b = L->n_b;
   pr.mu = L->mu; pr.lam = L->lam; pr.nu = 0.5 * L->lam / (L->lam + L->mu);
   hipStream_t st = (hipStream_t)stream;
@@ -469,9 +526,7 @@ int mhs_gather(const double* scratch, double* out, int64_t n_lead, int64_t n_ele
   if (!scratch || !out || !ptr || !idx) return fail("mhs_gather: null device pointer");
   const int64_t total = sb ? n_lead * n_nodes * n_a * n_b : n_lead * n_nodes * tail;
   if (total <= 0) return 0;
-  const int64_t blocks = (total + 255) / 256;
-  if (blocks > 0x7fffffffLL) return fail("mhs_gather: too large");
-  hipLaunchKernelGGL(k_gather, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, scratch, out,
+  hipLaunchKernelGGL(k_gather, dim3(grid_1d(total)), dim3(256), 0, (hipStream_t)stream, scratch, out,
                      n_lead, n_elem_cols, n_nodes, tail, ptr, idx, sb, n_a, n_b);
   return check(hipGetLastError(), "k_gather launch");
 }
@@ -480,7 +535,7 @@ int mhs_strain(double* buf, int64_t n_obs, int64_t n_cols, double mu, double lam
   if (!buf) return fail("mhs_strain: null device pointer");
   const int64_t total = n_obs * n_cols * 3;
   if (total <= 0) return 0;
-  hipLaunchKernelGGL(k_strain, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, (hipStream_t)stream, buf,
+  hipLaunchKernelGGL(k_strain, dim3(grid_1d(total)), dim3(256), 0, (hipStream_t)stream, buf,
                      n_obs, n_cols, mu, lam);
   return check(hipGetLastError(), "k_strain launch");
 }
@@ -490,11 +545,17 @@ int mhs_histogram(const mhs_launch* L, unsigned long long* counts, void* stream)
   Params pr{};
   pr.obs = L->obs; pr.src = L->src; pr.n_obs = L->n_obs; pr.n_src = L->n_src; pr.n_cols = L->n_cols;
   pr.obs_lo = L->obs_lo; pr.flags = L->flags;
-  const int64_t blocks = (L->obs_hi - L->obs_lo) * ((L->n_cols + 255) / 256);
-  if (blocks <= 0) return 0;
-  if (blocks > 0x7fffffffLL) return fail("mhs_histogram: too large");
-  hipLaunchKernelGGL(k_hist, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, pr, L->order, counts);
-  return check(hipGetLastError(), "k_hist launch");
+  const int64_t nblk = (L->n_cols + 255) / 256;
+  if (L->obs_hi <= L->obs_lo || nblk <= 0) return 0;
+  if (nblk * 256 > 0xffffffffLL) return fail("mhs_histogram: too many columns for one launch");
+  for (int64_t lo = L->obs_lo; lo < L->obs_hi; lo += MHS_ROWS_PER_LAUNCH) {
+    const int64_t rows = (L->obs_hi - lo) < MHS_ROWS_PER_LAUNCH ? (L->obs_hi - lo) : MHS_ROWS_PER_LAUNCH;
+    pr.obs_lo = lo;
+    hipLaunchKernelGGL(k_hist, dim3((unsigned)nblk, (unsigned)rows), dim3(256), 0, (hipStream_t)stream, pr,
+                       L->order, counts);
+    if (check(hipGetLastError(), "k_hist launch")) return -1;
+  }
+  return 0;
 }
 
 }  // extern "C"

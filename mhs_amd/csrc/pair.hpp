@@ -7,7 +7,7 @@
 #include "gen_image.inc"
 
 enum Kind { K_DISP = 0, K_STRESS = 1, K_EIGEN = 2, K_TRACTION = 3, K_INTERACTION = 4 };
-enum Flag { F_FOLD = 1, F_SUBTRACT = 2, F_PARTS = 4 };
+enum Flag { F_FOLD = 1, F_SUBTRACT = 2, F_PARTS = 4, F_MIXED = 8 };
 
 struct Params {
   const double* obs;     // n_obs x 3
@@ -22,6 +22,7 @@ struct Params {
   double* out2;
   int64_t n_obs, n_src, n_cols;
   int64_t obs_lo;        // first observer row of this launch
+  int64_t out_lo;        // observer row held by the first row of out / out2
   int kind, flags, n_a, n_b;
   double mu, lam, nu;
 };
@@ -240,7 +241,7 @@ MHS_INLINE void emit_stress(const Params& pr, int64_t o, int64_t col, int64_t s,
         }
         E[(a * 3 + b) * 3 + j] = e;
       }
-  const int64_t oo = o - pr.obs_lo;
+  const int64_t oo = o - pr.out_lo;
   if (pr.kind == K_STRESS || pr.kind == K_EIGEN) {
     const bool sub = (pr.flags & F_SUBTRACT) != 0;
 #pragma unroll
@@ -299,7 +300,7 @@ MHS_INLINE void emit_stress(const Params& pr, int64_t o, int64_t col, int64_t s,
 }
 
 MHS_INLINE void emit_disp(const Params& pr, int64_t o, int64_t col, const double* U, bool add) {
-  const int64_t oo = o - pr.obs_lo;
+  const int64_t oo = o - pr.out_lo;
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll

@@ -21,9 +21,14 @@ at order P0 in the stress family (the interaction workload):
   22 table accumulations 44                                     -> 85
 * far image R-family, per Gauss point: 28 + monomials to degree 5 20 +
   R^-3..R^-9 5 + 49 table accumulations 98                      -> 151
-* Q-family, per Gauss point: point, weight, offsets, R2, 1/R2, 1/Q 32;
-  power ladders 18; 35 D-monomials (with the weight) 60; 32 radial parts 32;
-  236 monomials 236; 1134 record FMAs 2268                       -> 2646
+* Q-family, contracted (interaction, one or two outputs), per Gauss point:
+  barycentric point, weight and offsets 26 and the FP64 accumulation 2 (FP64
+  in both modes); R, 1/R, 1/Q 10; the blocked monomial sum of
+  csrc/gen_image.inc q_blocked: 44 chain products, 22 first-use / Horner
+  products, 166 FMAs -> 398                                      -> 436
+  (in the mixed mode 28 of them FP64, 408 FP32); per pair: output functional
+  G 160, kappa 1134 FMAs 2268, budget-law order (triangle distance) ~170
+                                                                 -> 2600
 * per pair, closed-form near direct (three edges, solid angle, recursion)
   ~2.6e3; near image ~5.2e3; direct lift + contraction + rotation ~3.0e3;
   R-family lift + records + contraction ~6.0e3; epilogue 2e2.
@@ -39,7 +44,9 @@ from . import defaults, geometry
 
 FLOP_DIRECT_PT = 85
 FLOP_IMAGE_PT = 151
-FLOP_Q_PT = 2646
+FLOP_Q_PT = 436
+FLOP_Q_PT_FP64_IN_MIXED = 28
+FLOP_Q_PAIR = 2600
 FLOP_DIRECT_NEAR = 2600
 FLOP_IMAGE_NEAR = 5200
 FLOP_DIRECT_LIFT = 3000
@@ -120,5 +127,6 @@ def flops(h: dict) -> dict:
             + h.get("mirror_near", 0) * FLOP_DIRECT_NEAR + h.get("mirror_far12", 0) * 144 * FLOP_DIRECT_PT
             + h.get("mirror_far8", 0) * 64 * FLOP_DIRECT_PT + h["fold_pairs"] * FLOP_DIRECT_LIFT
             + (h["pairs"] - h["fold_pairs"]) * FLOP_IMAGE_LIFT)
-    q = h["q_points"] * FLOP_Q_PT + h["q_pairs"] * 500
-    return {"pair": float(pair), "q": float(q)}
+    q = h["q_points"] * FLOP_Q_PT + h["q_pairs"] * FLOP_Q_PAIR
+    q_fp32_mixed = h["q_points"] * (FLOP_Q_PT - FLOP_Q_PT_FP64_IN_MIXED)
+    return {"pair": float(pair), "q": float(q), "q_fp32_in_mixed": float(q_fp32_mixed)}

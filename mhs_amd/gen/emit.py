@@ -193,12 +193,24 @@ def emit_q(em: Emitter, name, records, n_ix):
 def emit_q_kappa(em: Emitter, records, n_ix):
     """Contracted Q-family: per pair, kappa_k = sum over the records of
     monomial k of C[rec] * G[slot] (G = the pair's output functional on the
-    81 DDG slots); per quadrature point, sum_k kappa_k * monomial_k."""
+    81 DDG slots); per quadrature point, sum_k kappa_k * monomial_k with
+    monomial_k = d1^a d2^b d3^c R^-n Q^-q.
+
+    The point sum is emitted register-blocked over NPT points that share each
+    kappa read: the spatial monomials d1^a d2^b d3^c are walked once in an
+    x-major chain (one multiply each), each feeds its radial groups' running
+    sums acc[(n, q)], and the groups are combined by nested Horner in 1/Q then
+    1/R.  kappa is laid out in that walk order."""
     by_mono = defaultdict(list)
     for r, rec in enumerate(records):
         slot = slot_of(rec[:n_ix])
         by_mono[tuple(rec[n_ix:n_ix + 5])].append((slot, r))
-    monos = sorted(by_mono)
+    monos = list(by_mono)
+    groups = sorted({m[3:] for m in monos})
+    gidx = {g: i for i, g in enumerate(groups)}
+    # x-major walk over the spatial parts
+    dparts = sorted({m[:3] for m in monos})
+    monos.sort(key=lambda m: (m[:3], gidx[m[3:]]))
     base = em.q_ddg_base
     ptr = [0]
     slots, coefs = [], []
@@ -207,39 +219,114 @@ def emit_q_kappa(em: Emitter, records, n_ix):
             slots.append(slot)
             coefs.append(base + r)
         ptr.append(len(slots))
+    kidx = {m: i for i, m in enumerate(monos)}
     em.w(f"#define MHS_QM {len(monos)}")
     em.w(f"#define MHS_QREC {len(slots)}")
+    em.w(f"#define MHS_QG {len(groups)}")
     em.w(f"__constant__ short mhs_qm_ptr[{len(ptr)}] = {{{', '.join(map(str, ptr))}}};")
     em.w(f"__constant__ short mhs_qm_coef[{len(coefs)}] = {{{', '.join(map(str, coefs))}}};")
     em.w(f"__constant__ unsigned char mhs_qm_slot[{len(slots)}] = {{{', '.join(map(str, slots))}}};")
-    em.w("// sum_k kap[o * MHS_QM + k] * monomial_k, per output o, into acc[o] (times w).")
-    em.w("template <int NOUT>")
-    em.w("__host__ __device__ __forceinline__ void q_kappa_point(double d1, double d2, double d3,")
-    em.w("    double ir, double iq, double w, const double* kap, double* acc)")
+    em.w("// s[o * NPT + j] = sum_k kap[o * MHS_QM + k] * monomial_k(point j), for")
+    em.w("// NOUT kappa sets and NPT points (d1, d2, d3, 1/R, 1/Q per point).")
+    em.w("template <typename T, int NPT, int NOUT>")
+    em.w("__device__ __forceinline__ void q_blocked(const T* __restrict__ kap, const T* x, const T* y,")
+    em.w("    const T* z, const T* ir, const T* iq, T* s)")
     em.w("{")
-    emax = [max(m[i] for m in monos) for i in range(5)]
-    for nm, var, e in (("x", "d1", emax[0]), ("y", "d2", emax[1]), ("zz", "d3", emax[2]),
-                       ("r", "ir", emax[3]), ("q", "iq", emax[4])):
-        em.w(f"  const double {nm}1 = {var};")
-        for k in range(2, e + 1):
-            em.w(f"  const double {nm}{k} = {nm}{k - 1} * {var};")
-    dparts = sorted({m[:3] for m in monos})
-    rparts = sorted({m[3:] for m in monos})
-    for a, b, c in dparts:
-        f = [f"x{a}"] * (a > 0) + [f"y{b}"] * (b > 0) + [f"zz{c}"] * (c > 0)
-        em.w(f"  const double D{a}{b}{c} = {' * '.join(f) if f else '1.0'};")
-    for n, q in rparts:
-        f = [f"r{n}"] * (n > 0) + [f"q{q}"] * (q > 0)
-        em.w(f"  const double R{n}{q} = {' * '.join(f) if f else '1.0'};")
-    em.w("  double s[NOUT];")
+    em.w(f"  T acc[NOUT][NPT][{len(groups)}];")
+    em.w("  T xa[NPT], xy[NPT], d[NPT];")
+    init = set()
+    by_d = defaultdict(list)
+    for m in monos:
+        by_d[m[:3]].append(m)
+    amax = max(a for a, _, _ in dparts)
+    for a in range(amax + 1):
+        da = [p for p in dparts if p[0] == a]
+        if not da:
+            continue
+        if a == 1:
+            em.w("#pragma unroll")
+            em.w("  for (int j = 0; j < NPT; ++j) xa[j] = x[j];")
+        elif a > 1:
+            em.w("#pragma unroll")
+            em.w("  for (int j = 0; j < NPT; ++j) xa[j] *= x[j];")
+        bmax = max(p[1] for p in da)
+        for b in range(bmax + 1):
+            db = [p for p in da if p[1] == b]
+            xa_one = a == 0
+            if b == 0:
+                xy_expr = None if xa_one else "xa[j]"
+            elif b == 1:
+                xy_expr = "y[j]" if xa_one else "xa[j] * y[j]"
+            else:
+                xy_expr = "xy[j] * y[j]"
+            xy_one = xy_expr is None
+            if xy_expr is not None:
+                em.w("#pragma unroll")
+                em.w(f"  for (int j = 0; j < NPT; ++j) xy[j] = {xy_expr};")
+            if not db:
+                continue
+            cmax = max(p[2] for p in db)
+            for c in range(cmax + 1):
+                if c == 0:
+                    one = xy_one
+                    src = None if one else "xy"
+                else:
+                    one = False
+                    if c == 1:
+                        expr = "z[j]" if xy_one else "xy[j] * z[j]"
+                    else:
+                        expr = "d[j] * z[j]"
+                    em.w("#pragma unroll")
+                    em.w(f"  for (int j = 0; j < NPT; ++j) d[j] = {expr};")
+                    src = "d"
+                ms = by_d.get((a, b, c), [])
+                if not ms:
+                    continue
+                em.w("#pragma unroll")
+                em.w("  for (int o = 0; o < NOUT; ++o) {")
+                for m in ms:
+                    em.w(f"    const T k{kidx[m]} = kap[o * {len(monos)} + {kidx[m]}];")
+                em.w("#pragma unroll")
+                em.w("    for (int j = 0; j < NPT; ++j) {")
+                for m in ms:
+                    g = gidx[m[3:]]
+                    k = f"k{kidx[m]}"
+                    tgt = f"acc[o][j][{g}]"
+                    if g not in init:
+                        em.w(f"      {tgt} = {k if one else f'{k} * {src}[j]'};")
+                    else:
+                        em.w(f"      {tgt} = {f'{tgt} + {k}' if one else f'fma({k}, {src}[j], {tgt})'};")
+                em.w("    }")
+                em.w("  }")
+                for m in ms:
+                    init.add(gidx[m[3:]])
+    assert len(init) == len(groups)
+    # combine: S = sum_n R^n sum_q Q^q acc[(n, q)], nested Horner
+    ns = sorted({g[0] for g in groups})
+    assert ns == list(range(ns[0], ns[-1] + 1)) and ns[0] == 0
+    qmins = {n: min(q for nn, q in groups if nn == n) for n in ns}
     em.w("#pragma unroll")
-    em.w("  for (int o = 0; o < NOUT; ++o) s[o] = 0.0;")
-    for k, (a, b, c, n, q) in enumerate(monos):
-        em.w(f"  {{ const double mm = D{a}{b}{c} * R{n}{q};")
-        em.w("#pragma unroll")
-        em.w(f"    for (int o = 0; o < NOUT; ++o) s[o] = fma(kap[o * {len(monos)} + {k}], mm, s[o]); }}")
+    em.w("  for (int j = 0; j < NPT; ++j) {")
+    em.w("    const T q1 = iq[j];")
+    maxpow = max(qmins.values())
+    for k in range(2, maxpow + 1):
+        em.w(f"    const T q{k} = q{k - 1} * iq[j];")
     em.w("#pragma unroll")
-    em.w("  for (int o = 0; o < NOUT; ++o) acc[o] = fma(w, s[o], acc[o]);")
+    em.w("    for (int o = 0; o < NOUT; ++o) {")
+    em.w("      T S, t;")
+    for n in reversed(ns):
+        qs = sorted(q for nn, q in groups if nn == n)
+        assert qs == list(range(qs[0], qs[-1] + 1))
+        em.w(f"      t = acc[o][j][{gidx[(n, qs[-1])]}];")
+        for q in reversed(qs[:-1]):
+            em.w(f"      t = fma(t, q1, acc[o][j][{gidx[(n, q)]}]);")
+        if n == ns[-1]:
+            em.w(f"      S = t * q{qmins[n]};")
+        else:
+            em.w(f"      S = fma(S, ir[j], t * q{qmins[n]});")
+    em.w("      s[o * NPT + j] = S;")
+    em.w("    }")
+    em.w("  }")
     em.w("}")
     em.w()
 

@@ -15,6 +15,41 @@ from . import _lib, geometry
 
 ROW_SCRATCH_BYTES = 4 << 30      # continuous-layout element-major scratch per launch
 
+PRECISIONS = ("fp64", "mixed")
+_precision = "fp64"
+
+
+def set_precision(mode: str) -> None:
+    """Arithmetic of later calls: "fp64" (default; every branch in double) or
+    "mixed" (far-field quadrature points in FP32 with FP64 accumulation; the
+    near field, fold and eigenstress stay FP64)."""
+    global _precision
+    if mode not in PRECISIONS:
+        raise ValueError(f"precision must be one of {PRECISIONS}, got {mode!r}")
+    _precision = mode
+
+
+def get_precision() -> str:
+    return _precision
+
+
+class precision:
+    """``with mhs_amd.precision("mixed"): ...`` -- scoped :func:`set_precision`."""
+
+    def __init__(self, mode: str):
+        if mode not in PRECISIONS:
+            raise ValueError(f"precision must be one of {PRECISIONS}, got {mode!r}")
+        self.mode, self.prev = mode, None
+
+    def __enter__(self):
+        self.prev = get_precision()
+        set_precision(self.mode)
+        return self
+
+    def __exit__(self, *exc):
+        set_precision(self.prev)
+        return False
+
 
 @dataclass
 class Job:
@@ -30,6 +65,7 @@ class Job:
     subtract: bool = True
     parts: bool = False
     strain: bool = False
+    mixed: bool = False         # FP32 far-field points, FP64 accumulation
     nrm: np.ndarray | None = None      # (n_obs, 3)
     rb: np.ndarray | None = None       # (n_obs, n_a, 3)
     sb: np.ndarray | None = None       # (n_src | n_nodes, n_b, 3)
@@ -181,7 +217,7 @@ class Prepared:
             self.fold_err = torch.zeros(1, dtype=torch.int32, device=dev.dev)
         self.kind = _lib.KIND[job.kind]
         self.flags = ((_lib.F_FOLD if job.fold else 0) | (_lib.F_SUBTRACT if job.subtract else 0)
-                      | (_lib.F_PARTS if job.parts else 0))
+                      | (_lib.F_PARTS if job.parts else 0) | (_lib.F_MIXED if job.mixed else 0))
         self.n_b_launch = 3 if (cont and job.kind == "interaction") else job.n_b
 
     def _launch_struct(self, lo, hi, out_p, out2_p, stages):

@@ -164,20 +164,42 @@ def _q_group(obs, tri, fr, order, nu, scale_c, eps, nq, want, K):
     ir = 1.0 / r2
     iq = 1.0 / (r2 - d3)
     basis = np.empty((len(monos), obs.shape[0], K))
+    # power ladders by repeated products (image.py:299-312 caches them too)
+    ladders = {}
+
+    def power(name, base, e):
+        lad = ladders.setdefault(name, [None, base])
+        while len(lad) <= e:
+            lad.append(lad[-1] * base)
+        return lad[e]
+
+    spatial, radial = {}, {}
     for k, (a, b, c, n, q) in enumerate(monos):
-        t = np.ones_like(r2)
-        if a:
-            t = t * d1 ** a
-        if b:
-            t = t * d2 ** b
-        if c:
-            t = t * d3 ** c
-        if n > 0:
-            t = t * ir ** n
-        elif n < 0:
-            t = t * r2 ** (-n)
-        if q:
-            t = t * iq ** q
+        ds = spatial.get((a, b, c))
+        if ds is None:
+            ds = None
+            for nm, base, e in (("d1", d1, a), ("d2", d2, b), ("d3", d3, c)):
+                if e:
+                    ds = power(nm, base, e) if ds is None else ds * power(nm, base, e)
+            spatial[(a, b, c)] = ds
+        rq = radial.get((n, q))
+        if rq is None:
+            rq = None
+            if n > 0:
+                rq = power("ir", ir, n)
+            elif n < 0:
+                rq = power("r", r2, -n)
+            if q:
+                rq = power("iq", iq, q) if rq is None else rq * power("iq", iq, q)
+            radial[(n, q)] = rq
+        if ds is None and rq is None:
+            t = np.ones_like(r2)
+        elif ds is None:
+            t = rq
+        elif rq is None:
+            t = ds
+        else:
+            t = ds * rq
         basis[k] = t @ wN
     dg = ddg = None
     if "U" in want:

@@ -194,13 +194,19 @@ def weighted_far(fr: Frame, obs, eps, order, need0, ng):
     for n in range(1, max(need0) + 1, 2):
         rpow[n] = cur
         cur = cur * inv2
+    dmax = max(need0.values())
+    p0, p1 = [np.ones_like(xi0)], [np.ones_like(xi1)]
+    for _ in range(dmax):
+        p0.append(p0[-1] * xi0)
+        p1.append(p1[-1] * xi1)
     W = {}
     for n, d in need0.items():
         Wn = np.zeros((X.shape[0], Nq.shape[1], d + 1, d + 1))
+        wr = wq[None, :] * rpow[n]
         for a in range(d + 1):
+            wa = wr * p0[a]
             for b in range(d + 1 - a):
-                mono = wq[None, :] * xi0 ** a * xi1 ** b
-                Wn[:, :, a, b] = (mono * rpow[n]) @ Nq
+                Wn[:, :, a, b] = (wa * p1[b]) @ Nq
         W[n] = Wn
     return W, z, X
 

@@ -178,3 +178,27 @@ def test_float32_out_and_device_reuse(M, small):
     r = M.disp_matrix(ob, tb, mat, eb, out=out)
     assert r is out
     assert relmax(out, g["disp_b_p0"]) < 1e-6
+
+
+def test_c3_full_matrix_one_call(M):
+    """The whole n = 10k strike/strike matrix in one call (10^8 pairs in one
+    set of launches) against the golden row blocks, so every row chunk and
+    column tile of the launch geometry is exercised; then the mixed mode
+    against FP64 over all 10^8 entries (stated bound 1e-6)."""
+    from mhs_amd import workloads
+    tris, eps = workloads.c3_interaction()
+    g = np.load(GOLDEN / "c3_rows.npz")
+    mat = M.Material(30.0, 30.0)
+    print("case C3 full matrix fp64", flush=True)
+    K = M.interaction_matrix(tris, mat, eps, receiver="strike", source="strike")
+    for lo in (0, 5000, 5024, 9998):
+        assert relmax(K[lo:lo + 2], g[f"rows_{lo}"]) < TOL, lo
+    print("case C3 full matrix mixed", flush=True)
+    with M.precision("mixed"):
+        Km = M.interaction_matrix(tris, mat, eps, receiver="strike", source="strike")
+    assert M.get_precision() == "fp64"
+    err = relmax(Km, K)
+    print(f"mixed relmax vs fp64 {err:.3e}", flush=True)
+    assert err < 1e-6
+    for lo in (0, 5000, 5024, 9998):
+        assert relmax(Km[lo:lo + 2], g[f"rows_{lo}"]) < 1e-6, lo

@@ -140,3 +140,18 @@ def test_component_index():
     assert list(M.tdcs.component_index(("strike", "normal"))) == [0, 2]
     with pytest.raises(ValueError):
         M.tdcs.component_index(())
+
+
+def test_precision_setting():
+    import mhs_amd
+    assert mhs_amd.get_precision() == "fp64"
+    with mhs_amd.precision("mixed"):
+        assert mhs_amd.get_precision() == "mixed"
+        with mhs_amd.precision("fp64"):
+            assert mhs_amd.get_precision() == "fp64"
+        assert mhs_amd.get_precision() == "mixed"
+    assert mhs_amd.get_precision() == "fp64"
+    with pytest.raises(ValueError):
+        mhs_amd.set_precision("fp16")
+    with pytest.raises(ValueError):
+        mhs_amd.precision("half")
