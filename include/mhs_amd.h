@@ -48,7 +48,7 @@ typedef struct mhs_launch {
   int32_t abi_version;      /* MHS_ABI_VERSION */
   int32_t kind;             /* MHS_DISP .. MHS_INTERACTION */
   int32_t order;            /* nodal order 0, 1, 2 (K = 1, 3, 6 columns per source) */
-  int32_t flags;            /* MHS_FOLD | MHS_SUBTRACT | MHS_PARTS */
+  int32_t flags;            /* MHS_FOLD | MHS_SUBTRACT | MHS_PARTS | MHS_MIXED */
   int32_t n_a, n_b;         /* interaction receiver / source component counts */
   int64_t n_obs, n_src, n_cols;   /* n_cols = n_src * K (element-major columns) */
   int64_t obs_lo, obs_hi;   /* observer rows [obs_lo, obs_hi) of this launch; out row 0 = obs_lo */
