@@ -39,7 +39,8 @@ enum SrcOff {
   S_CREASE = 51, S_CREASE_R = 53, S_DIAM = 54, S_STRADDLE = 55,
   S_IP = 56,    // reflected triangle in-plane coords (3 x 2)
   S_IABC = 62,  // reflected triangle affine rows
-  S_IAREA = 71, S_IL = 72
+  S_IAREA = 71, S_IL = 72,
+  S_RC = 73     // largest vertex distance from the centroid (bounding sphere)
 };
 
 // Frame in registers.
@@ -612,6 +613,14 @@ MHS_INLINE void weighted_far(const Frame& f, int k, double X1, double X2, double
   }
 }
 
+// sqrt(|o - c|^2 + eps^2), the near/far switch distance (moments.py:551-554),
+// with the rounding pinned by explicit FMAs so that every kernel that
+// classifies a pair (the near pass, the far pass) takes the same branch.
+MHS_INLINE double centroid_distance(const double* c, const double o[3], double eps) {
+  const double d0 = o[0] - c[0], d1 = o[1] - c[1], d2 = o[2] - c[2];
+  return sqrt(fma(d0, d0, fma(d1, d1, fma(d2, d2, eps * eps))));
+}
+
 // Weighted tables of node k by the hybrid switch (moments.py:551-601);
 // returns the plane height z.
 template <class S, int P>
@@ -625,8 +634,7 @@ MHS_INLINE double weighted_tables(const Frame& f, const double* __restrict__ v, 
   const double X2 = d0 * f.e2[0] + d1 * f.e2[1] + d2 * f.e2[2];
   X1o = X1; X2o = X2;
   const double h2 = z * z + eps * eps;
-  const double D2 = d0 * d0 + d1 * d1 + d2 * d2 + eps * eps;
-  const double Dn = sqrt(D2);
+  const double Dn = centroid_distance(f.c, o, eps);
   if (Dn <= 10.0 * f.L) {
     double M[6][36];
     raw_moments_near<S>(f, v, X1, X2, h2, M);

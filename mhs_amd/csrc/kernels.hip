@@ -18,11 +18,17 @@
 #include "fold.hpp"
 #include "../../include/mhs_amd.h"
 
-#ifndef MHS_QC_NPT
-#define MHS_QC_NPT 4   // quadrature points per lane in the contracted Q pass
+#ifndef MHS_FAR_NPT32
+#define MHS_FAR_NPT32 4   // quadrature points per lane of the far pass, FP32 points
+#endif
+#ifndef MHS_FAR_NPT64
+#define MHS_FAR_NPT64 2   // quadrature points per lane of the far pass, FP64 points
 #endif
 #ifndef MHS_QC_WAVES
-#define MHS_QC_WAVES 2 // occupancy target of the contracted Q pass (waves per SIMD)
+#define MHS_QC_WAVES 3 // occupancy target of the far pass (waves per SIMD)
+#endif
+#ifndef MHS_QC_CT
+#define MHS_QC_CT 8    // column tiles per block of the contracted Q pass
 #endif
 
 static thread_local char g_err[512];
@@ -66,120 +72,321 @@ __global__ void __launch_bounds__(256) k_q(Params pr) {
 }
 
 
-// ------------------------------------------------- contracted Q-family --
-// Interaction with one or two output components: the Q-family of a pair
-// collapses to sum_k kappa_k * monomial_k, kappa = the records contracted with
-// the pair's output functional G[slot] = E_a[i][p] F_b[j][m] (receiver and
-// source halves of the traction-operator / Hooke / frame chain).
+// ------------------------------------------- contracted interaction, far --
+// Interaction with one or two output components (receiver frame row a,
+// source frame row b).  Every far-field term of a pair is a quadrature sum of
+// a pointwise kernel, so each is contracted with the pair's output functional
+// once per pair and the point loop sums one scalar series:
 //
-// LPP = 64 / NPT lanes per pair, NPT quadrature points per lane per pass, so
-// one pass covers 64 points (the budget law's floor of 8 x 8).  kappa lives in
-// LDS and every kappa read feeds NPT FMAs.  T is the point arithmetic (double,
-// or float in the mixed mode); the point geometry, the weights and the
-// per-lane and cross-lane sums stay in double.
-template <typename T, int P, int NOUT, int NPT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MHS_QC_WAVES)))
-k_qc(Params pr) {
-  constexpr int K = (P + 1) * (P + 2) / 2;
+//   direct (Kelvin-Cortez, source triangle, far when sqrt(|x-c|^2+eps^2) > 10 L):
+//     -C1 sum w N [A3/R^3 + (A5c c + A5a a + A5b b + A5e)/R^5
+//                  + (15 a b + A7c c + A7e)/R^7],
+//     a = d.E.d, b = d.F.d, c = d.sym(EF).d, E the receiver half (traction
+//     along frame row a on the receiver plane, in strain space), F the source
+//     half (Hooke of slip along frame row b on the source plane), A7e the
+//     subtracted eigenstress (15 eps^4 / 8 pi) rb.F.nr / C1;
+//   image R-family (mirrored triangle, far by the same rule, not folded):
+//     sum w N sum_k kappaR_k d^abc R^-n, kappaR_k = sum_v F_v BR[k][v];
+//   image Q-family (not folded, budget-law order):
+//     sum w N sum_k kappaQ_k d^abc R^-n Q^-q, kappaQ_k = sum_v F_v BQ[k][v];
+//
+// BR, BQ (the image records contracted with E, BR with the receiver-depth
+// powers of its coefficients) are built once per block: a block is one
+// receiver row and MHS_QC_CT column tiles of PPB pairs.  LPP = 64 / NPT lanes
+// per pair, NPT points per lane per 64-point chunk; the kappas live in LDS and
+// every read feeds NPT FMAs.  T is the point arithmetic (double, or float in
+// the mixed mode); point geometry, weights and all sums stay in double.  The
+// near parts (closed forms, folded mirror) are added by k_near.
+#define MHS_DC 25   // per-output direct coefficients: E(6) F(6) sym(EF)(6) A3 A5a A5b A5e A7c A7e A5c
+
+template <typename T, int NPT, int NOUT>
+__device__ __forceinline__ void d_blocked(const T* __restrict__ kd, const T* x, const T* y, const T* z,
+                                          const T* ir, T* s) {
+  T p00[NPT], p11[NPT], p22[NPT], p01[NPT], p02[NPT], p12[NPT], i2[NPT], i3[NPT];
+#pragma unroll
+  for (int j = 0; j < NPT; ++j) {
+    p00[j] = x[j] * x[j]; p11[j] = y[j] * y[j]; p22[j] = z[j] * z[j];
+    p01[j] = (x[j] + x[j]) * y[j]; p02[j] = (x[j] + x[j]) * z[j]; p12[j] = (y[j] + y[j]) * z[j];
+    i2[j] = ir[j] * ir[j];
+    i3[j] = i2[j] * ir[j];
+  }
+#pragma unroll
+  for (int o = 0; o < NOUT; ++o) {
+    const T* c = kd + o * MHS_DC;
+    T cf[MHS_DC];
+#pragma unroll
+    for (int t = 0; t < MHS_DC; ++t) cf[t] = c[t];
+#pragma unroll
+    for (int j = 0; j < NPT; ++j) {
+      const T a = fma(cf[0], p00[j], fma(cf[1], p11[j], fma(cf[2], p22[j],
+                  fma(cf[3], p01[j], fma(cf[4], p02[j], cf[5] * p12[j])))));
+      const T b = fma(cf[6], p00[j], fma(cf[7], p11[j], fma(cf[8], p22[j],
+                  fma(cf[9], p01[j], fma(cf[10], p02[j], cf[11] * p12[j])))));
+      const T cc = fma(cf[12], p00[j], fma(cf[13], p11[j], fma(cf[14], p22[j],
+                   fma(cf[15], p01[j], fma(cf[16], p02[j], cf[17] * p12[j])))));
+      const T s5 = fma(cf[24], cc, fma(cf[19], a, fma(cf[20], b, cf[21])));
+      const T s7 = fma((T)15 * a, b, fma(cf[22], cc, cf[23]));
+      s[o * NPT + j] = fma(fma(s7, i2[j], s5), i2[j], cf[18]) * i3[j];
+    }
+  }
+}
+
+// One quadrature pass of a pair over ng x ng collapsed Gauss points of the
+// source triangle (IMG false: d = x - y) or of its mirror (IMG true:
+// d = x - y*), summing the selected series into acc_d (direct) / acc_i.
+template <typename T, int P, int NOUT, int NPT, bool IMG, bool DIR, bool RF, bool QF>
+__device__ __forceinline__ void far_pass(const Params& pr, const double* __restrict__ src, int k,
+                                         const double ob[3], int ng, int li, const T* kd, const T* kr,
+                                         const T* kq, double* acc_d, double* acc_i) {
   constexpr int LPP = 64 / NPT;
-  constexpr int PPB = 256 / LPP;
-  __shared__ T kap_s[PPB][NOUT * MHS_QM];
-  __shared__ double g_s[PPB][NOUT * 81];
-  const int grp = threadIdx.x / LPP, li = threadIdx.x % LPP;
-  const int64_t o = pr.obs_lo + blockIdx.y;
-  const int64_t col = (int64_t)blockIdx.x * PPB + grp;
-  bool valid = col < pr.n_cols;
-  int64_t s = 0;
-  int k = 0, nq = 0;
-  const double* src = pr.src;
-  double ob[3] = {0.0, 0.0, 0.0};
-  if (valid) {
-    s = col / K;
-    k = (int)(col - s * K);
-    src = pr.src + s * MHS_SRC_STRIDE;
-    ob[0] = pr.obs[3 * o]; ob[1] = pr.obs[3 * o + 1]; ob[2] = pr.obs[3 * o + 2];
-    valid = !in_fold_band(src, ob, pr.flags);
-    if (valid) nq = q_order<P>(src, ob);
-  }
-  if (valid) {
-    const double* nr = pr.nrm + 3 * o;
-    const double* n = src + S_N;
-    for (int t = li; t < NOUT * 81; t += LPP) {
-      const int out = t / 81, slot = t % 81;
-      const int a = out / pr.n_b, b = out % pr.n_b;
-      const double* rb = pr.rb + ((int64_t)o * pr.n_a + a) * 3;
-      const double* sb = pr.sb + ((int64_t)s * pr.n_b + b) * 3;
-      const int i = slot / 27, j = (slot / 9) % 3, p = (slot / 3) % 3, m = slot % 3;
-      const double rn = rb[0] * nr[0] + rb[1] * nr[1] + rb[2] * nr[2];
-      const double sn = sb[0] * n[0] + sb[1] * n[1] + sb[2] * n[2];
-      const double E = pr.mu * (rb[i] * nr[p] + rb[p] * nr[i]) + (i == p ? pr.lam * rn : 0.0);
-      const double F = pr.mu * (sb[j] * n[m] + n[j] * sb[m]) + (j == m ? pr.lam * sn : 0.0);
-      g_s[grp][t] = E * F;
-    }
-  }
-  __syncthreads();
-  if (valid) {
-    for (int t = li; t < NOUT * MHS_QM; t += LPP) {
-      const int out = t / MHS_QM, km = t % MHS_QM;
-      double acc = 0.0;
-      for (int r = mhs_qm_ptr[km]; r < mhs_qm_ptr[km + 1]; ++r)
-        acc = fma(pr.coef[mhs_qm_coef[r]], g_s[grp][out * 81 + mhs_qm_slot[r]], acc);
-      kap_s[grp][t] = (T)acc;
-    }
-  }
-  __syncthreads();
-  double acc[NOUT];
+  const double* v = src + S_V;
+  const double eps2 = src[S_EPS] * src[S_EPS];
+  const double ar2 = 2.0 * src[S_AREA];
+  const int off = ng * (ng - 1) / 2;
+  const double* gx = pr.gl_x + off;
+  const double* gw = pr.gl_w + off;
+  const int npts = ng * ng;
+  for (int base = 0; base < npts; base += 64) {
+    // compiler-only fence: keeps the kappa reads inside the chunk instead of
+    // hoisting all of them out of the loop into registers
+    __asm__ volatile("" ::: "memory");
+    T x[NPT], y[NPT], z[NPT], ir[NPT], iq[NPT], sv[NOUT * NPT];
+    double wn[NPT];
 #pragma unroll
-  for (int t = 0; t < NOUT; ++t) acc[t] = 0.0;
-  if (valid) {
-    const double* v = src + S_V;
-    const double eps2 = src[S_EPS] * src[S_EPS];
-    const double ar2 = 2.0 * src[S_AREA];
-    const int off = nq * (nq - 1) / 2;
-    const double* gx = pr.gl_x + off;
-    const double* gw = pr.gl_w + off;
-    const int npts = nq * nq;
-    for (int base = 0; base < npts; base += 64) {
-      // compiler-only fence: keeps the kappa reads inside the pass instead of
-      // hoisting all of them (2 * MHS_QM registers) out of the loop
-      __asm__ volatile("" ::: "memory");
-      T x[NPT], y[NPT], z[NPT], ir[NPT], iq[NPT], sv[NOUT * NPT];
-      double wn[NPT];
-#pragma unroll
-      for (int u = 0; u < NPT; ++u) {
-        const int g0 = base + u * LPP + li;
-        const bool in = g0 < npts;
-        const int g = in ? g0 : 0;
-        const int i = g / nq, j = g - i * nq;
-        const double gi = gx[i];
-        const double x1 = gi, x2 = gx[j] * (1.0 - gi);
-        const double w = in ? gw[i] * gw[j] * (1.0 - gi) * ar2 : 0.0;
-        const double l0 = 1.0 - x1 - x2;
-        const double y0 = l0 * v[0] + x1 * v[3] + x2 * v[6];
-        const double y1 = l0 * v[1] + x1 * v[4] + x2 * v[7];
-        const double y2 = l0 * v[2] + x1 * v[5] + x2 * v[8];
-        wn[u] = w * shape_at<P>(k, l0, x1, x2);
-        const T d1 = (T)(ob[0] - y0), d2 = (T)(ob[1] - y1), d3 = (T)(ob[2] + y2);
-        const T R = sqrt(d1 * d1 + d2 * d2 + d3 * d3 + (T)eps2);
-        x[u] = d1; y[u] = d2; z[u] = d3;
-        ir[u] = (T)1 / R;
-        iq[u] = (T)1 / (R - d3);
-      }
-      q_blocked<T, NPT, NOUT>(kap_s[grp], x, y, z, ir, iq, sv);
+    for (int u = 0; u < NPT; ++u) {
+      const int g0 = base + u * LPP + li;
+      const bool in = g0 < npts;
+      const int g = in ? g0 : 0;
+      const int i = g / ng, j = g - i * ng;
+      const double gi = gx[i];
+      const double x1 = gi, x2 = gx[j] * (1.0 - gi);
+      const double w = in ? gw[i] * gw[j] * (1.0 - gi) * ar2 : 0.0;
+      const double l0 = 1.0 - x1 - x2;
+      const double y0 = l0 * v[0] + x1 * v[3] + x2 * v[6];
+      const double y1 = l0 * v[1] + x1 * v[4] + x2 * v[7];
+      const double y2 = l0 * v[2] + x1 * v[5] + x2 * v[8];
+      wn[u] = w * shape_at<P>(k, l0, x1, x2);
+      const T d1 = (T)(ob[0] - y0), d2 = (T)(ob[1] - y1), d3 = (T)(IMG ? ob[2] + y2 : ob[2] - y2);
+      const T R = sqrt(d1 * d1 + d2 * d2 + d3 * d3 + (T)eps2);
+      x[u] = d1; y[u] = d2; z[u] = d3;
+      ir[u] = (T)1 / R;
+      iq[u] = QF ? (T)1 / (R - d3) : (T)0;
+    }
+    if (DIR) {
+      d_blocked<T, NPT, NOUT>(kd, x, y, z, ir, sv);
 #pragma unroll
       for (int t = 0; t < NOUT; ++t)
 #pragma unroll
-        for (int u = 0; u < NPT; ++u) acc[t] = fma(wn[u], (double)sv[t * NPT + u], acc[t]);
+        for (int u = 0; u < NPT; ++u) acc_d[t] = fma(wn[u], (double)sv[t * NPT + u], acc_d[t]);
+    }
+    if (RF) {
+      __asm__ volatile("" ::: "memory");
+      r_blocked<T, NPT, NOUT>(kr, x, y, z, ir, iq, sv);
+#pragma unroll
+      for (int t = 0; t < NOUT; ++t)
+#pragma unroll
+        for (int u = 0; u < NPT; ++u) acc_i[t] = fma(wn[u], (double)sv[t * NPT + u], acc_i[t]);
+    }
+    if (QF) {
+      __asm__ volatile("" ::: "memory");
+      q_blocked<T, NPT, NOUT>(kq, x, y, z, ir, iq, sv);
+#pragma unroll
+      for (int t = 0; t < NOUT; ++t)
+#pragma unroll
+        for (int u = 0; u < NPT; ++u) acc_i[t] = fma(wn[u], (double)sv[t * NPT + u], acc_i[t]);
     }
   }
-#pragma unroll
-  for (int t = 0; t < NOUT; ++t)
-    for (int off = LPP / 2; off > 0; off >>= 1) acc[t] += __shfl_xor(acc[t], off, LPP);
-  if (valid && li == 0) {
-    const int64_t oo = o - pr.out_lo;
-#pragma unroll
-    for (int t = 0; t < NOUT; ++t) pr.out[(oo * pr.n_cols + col) * NOUT + t] += acc[t];
+}
+
+template <typename T, int P, int NOUT, int NPT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MHS_QC_WAVES)))
+k_far(Params pr) {
+  constexpr int K = (P + 1) * (P + 2) / 2;
+  constexpr int LPP = 64 / NPT;
+  constexpr int PPB = 256 / LPP;
+  constexpr int QB = MHS_QM * 6, RB = MHS_RM * 6;
+  __shared__ double BQ_s[NOUT][QB];
+  __shared__ double BR_s[NOUT][RB];
+  __shared__ double E_s[NOUT][9];
+  __shared__ T kq_s[PPB][NOUT * MHS_QM];
+  __shared__ T kr_s[PPB][NOUT * MHS_RM];
+  __shared__ T kd_s[PPB][NOUT * MHS_DC];
+  const int tid = threadIdx.x;
+  const int grp = tid / LPP, li = tid % LPP;
+  const int n_a = pr.n_a, n_b = pr.n_b;
+  const int64_t o = pr.obs_lo + blockIdx.y;
+  const double ob[3] = {pr.obs[3 * o], pr.obs[3 * o + 1], pr.obs[3 * o + 2]};
+  const double* nr = pr.nrm + 3 * o;
+  // receiver half E_a (strain-space traction functional, frame row a)
+  if (tid < n_a * 9) {
+    const int a = tid / 9, i = (tid % 9) / 3, p = tid % 3;
+    const double* rb = pr.rb + ((int64_t)o * n_a + a) * 3;
+    const double rn = rb[0] * nr[0] + rb[1] * nr[1] + rb[2] * nr[2];
+    E_s[a][i * 3 + p] = pr.mu * (rb[i] * nr[p] + rb[p] * nr[i]) + (i == p ? pr.lam * rn : 0.0);
   }
+  __syncthreads();
+  for (int t = tid; t < n_a * QB; t += 256) {
+    const int a = t / QB, kv = t % QB;
+    double acc = 0.0;
+    for (int r = mhs_qb_ptr[kv]; r < mhs_qb_ptr[kv + 1]; ++r)
+      acc = fma(pr.coef[mhs_qb_coef[r]], E_s[a][mhs_qb_ip[r]], acc);
+    BQ_s[a][kv] = acc;
+  }
+  {
+    const double zp[3] = {1.0, ob[2], ob[2] * ob[2]};
+    for (int t = tid; t < n_a * RB; t += 256) {
+      const int a = t / RB, kv = t % RB;
+      double acc = 0.0;
+      for (int r = mhs_rb_ptr[kv]; r < mhs_rb_ptr[kv + 1]; ++r)
+        acc = fma(pr.coef[mhs_rb_coef[r]] * zp[mhs_rb_pz[r]], E_s[a][mhs_rb_ip[r]], acc);
+      BR_s[a][kv] = acc;
+    }
+  }
+  __syncthreads();
+  const double C1 = 1.0 / (16.0 * M_PI * pr.mu * (1.0 - pr.nu));
+  const double c34 = 3.0 - 4.0 * pr.nu;
+  for (int tile = 0; tile < MHS_QC_CT; ++tile) {
+    const int64_t col = ((int64_t)blockIdx.x * MHS_QC_CT + tile) * PPB + grp;
+    const bool valid = col < pr.n_cols;
+    int64_t s = 0;
+    int k = 0, nd = 0, ni = 0, nq = 0;
+    bool dfar = false, rfar = false, img = false;
+    const double* src = pr.src;
+    if (valid) {
+      s = col / K;
+      k = (int)(col - s * K);
+      src = pr.src + s * MHS_SRC_STRIDE;
+      const double eps = src[S_EPS];
+      const double Dd = centroid_distance(src + S_C, ob, eps);
+      dfar = !(Dd <= 10.0 * src[S_L]);
+      nd = Dd > 40.0 * src[S_L] ? 8 : 12;
+      img = !in_fold_band(src, ob, pr.flags);
+      if (img) {
+        const double Di = centroid_distance(src + S_IC, ob, eps);
+        rfar = !(Di <= 10.0 * src[S_IL]);
+        ni = Di > 40.0 * src[S_IL] ? 8 : 12;
+        nq = q_order_fast<P>(src, ob);
+      }
+      // source half F_b (Hooke on the source plane, slip along frame row b)
+      const double* n = src + S_N;
+      double F[2][6];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        if (b < n_b) {
+          const double* sb = pr.sb + (s * n_b + b) * 3;
+          const double sn = sb[0] * n[0] + sb[1] * n[1] + sb[2] * n[2];
+          F[b][0] = 2.0 * pr.mu * sb[0] * n[0] + pr.lam * sn;
+          F[b][1] = 2.0 * pr.mu * sb[1] * n[1] + pr.lam * sn;
+          F[b][2] = 2.0 * pr.mu * sb[2] * n[2] + pr.lam * sn;
+          F[b][3] = pr.mu * (sb[0] * n[1] + n[0] * sb[1]);
+          F[b][4] = pr.mu * (sb[0] * n[2] + n[0] * sb[2]);
+          F[b][5] = pr.mu * (sb[1] * n[2] + n[1] * sb[2]);
+        } else {
+#pragma unroll
+          for (int v = 0; v < 6; ++v) F[b][v] = 0.0;
+        }
+      }
+      if (img) {
+        for (int t = li; t < NOUT * MHS_QM; t += LPP) {
+          const int out = t / MHS_QM, km = t - out * MHS_QM;
+          const int a = out / n_b, b = out - a * n_b;
+          const double* Bk = &BQ_s[a][km * 6];
+          const double* Fb = F[b == 0 ? 0 : 1];
+          double kv = Fb[0] * Bk[0];
+#pragma unroll
+          for (int v = 1; v < 6; ++v) kv = fma(Fb[v], Bk[v], kv);
+          kq_s[grp][t] = (T)kv;
+        }
+      }
+      if (img && rfar) {
+        for (int t = li; t < NOUT * MHS_RM; t += LPP) {
+          const int out = t / MHS_RM, km = t - out * MHS_RM;
+          const int a = out / n_b, b = out - a * n_b;
+          const double* Bk = &BR_s[a][km * 6];
+          const double* Fb = F[b == 0 ? 0 : 1];
+          double kv = Fb[0] * Bk[0];
+#pragma unroll
+          for (int v = 1; v < 6; ++v) kv = fma(Fb[v], Bk[v], kv);
+          kr_s[grp][t] = (T)kv;
+        }
+      }
+      if (dfar && li < NOUT) {
+        const int a = li / n_b, b = li - (li / n_b) * n_b;
+        if (a < n_a) {
+          const double* Ea = E_s[a];
+          const double* Fv = F[b == 0 ? 0 : 1];
+          const double Fm[9] = {Fv[0], Fv[3], Fv[4], Fv[3], Fv[1], Fv[5], Fv[4], Fv[5], Fv[2]};
+          double M[9];
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+              M[i * 3 + j] = Ea[i * 3] * Fm[j] + Ea[i * 3 + 1] * Fm[3 + j] + Ea[i * 3 + 2] * Fm[6 + j];
+          double EF = 0.0;
+#pragma unroll
+          for (int t = 0; t < 9; ++t) EF = fma(Ea[t], Fm[t], EF);
+          const double trE = Ea[0] + Ea[4] + Ea[8], trF = Fm[0] + Fm[4] + Fm[8];
+          const double eps = src[S_EPS], e2 = eps * eps;
+          const double* rb = pr.rb + ((int64_t)o * n_a + a) * 3;
+          double rFn = 0.0;
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+            rFn += rb[i] * (Fm[i * 3] * nr[0] + Fm[i * 3 + 1] * nr[1] + Fm[i * 3 + 2] * nr[2]);
+          T* c = &kd_s[grp][li * MHS_DC];
+          c[0] = (T)Ea[0]; c[1] = (T)Ea[4]; c[2] = (T)Ea[8];
+          c[3] = (T)Ea[1]; c[4] = (T)Ea[2]; c[5] = (T)Ea[5];
+          c[6] = (T)Fv[0]; c[7] = (T)Fv[1]; c[8] = (T)Fv[2];
+          c[9] = (T)Fv[3]; c[10] = (T)Fv[4]; c[11] = (T)Fv[5];
+          c[12] = (T)M[0]; c[13] = (T)M[4]; c[14] = (T)M[8];
+          c[15] = (T)(0.5 * (M[1] + M[3])); c[16] = (T)(0.5 * (M[2] + M[6])); c[17] = (T)(0.5 * (M[5] + M[7]));
+          c[18] = (T)((1.0 - c34) * EF + trE * trF);
+          c[19] = (T)(-3.0 * trF);
+          c[20] = (T)(-3.0 * trE);
+          c[21] = (T)(-6.0 * (1.0 - pr.nu) * e2 * EF);
+          c[22] = (T)(30.0 * (1.0 - pr.nu) * e2);
+          c[23] = (T)((pr.flags & F_SUBTRACT) ? 15.0 * e2 * e2 / (8.0 * M_PI * C1) * rFn : 0.0);
+          c[24] = (T)(3.0 * c34 - 9.0);
+        }
+      }
+    }
+    __syncthreads();
+    double acc_d[NOUT], acc_i[NOUT];
+#pragma unroll
+    for (int t = 0; t < NOUT; ++t) acc_d[t] = acc_i[t] = 0.0;
+    const T* kd = kd_s[grp];
+    const T* kr = kr_s[grp];
+    const T* kq = kq_s[grp];
+    if (valid && dfar)
+      far_pass<T, P, NOUT, NPT, false, true, false, false>(pr, src, k, ob, nd, li, kd, kr, kq, acc_d, acc_i);
+    // the R and Q series run as separate passes even at equal orders: summed
+    // in one pass their registers no longer fit two waves per SIMD
+    if (valid && img && rfar)
+      far_pass<T, P, NOUT, NPT, true, false, true, false>(pr, src, k, ob, ni, li, kd, kr, kq, acc_d, acc_i);
+    if (valid && img)
+      far_pass<T, P, NOUT, NPT, true, false, false, true>(pr, src, k, ob, nq, li, kd, kr, kq, acc_d, acc_i);
+    double res[NOUT];
+#pragma unroll
+    for (int t = 0; t < NOUT; ++t) {
+      res[t] = fma(-C1, acc_d[t], acc_i[t]);
+      for (int off = LPP / 2; off > 0; off >>= 1) res[t] += __shfl_xor(res[t], off, LPP);
+    }
+    if (valid && li == 0) {
+      const int64_t oo = o - pr.out_lo;
+#pragma unroll
+      for (int t = 0; t < NOUT; ++t) pr.out[(oo * pr.n_cols + col) * NOUT + t] = res[t];
+    }
+    __syncthreads();     // the kappas are rewritten by the next tile
+  }
+}
+
+// Near parts of the contracted interaction (closed forms, folded mirror),
+// added after k_far; a thread per pair, most of which have nothing near.
+template <int P>
+__global__ void __launch_bounds__(256) k_near(Params pr) {
+  const int64_t o = pr.obs_lo + blockIdx.y;
+  const int64_t col = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (col >= pr.n_cols) return;
+  pair_near_item<P>(pr, o, col);
 }
 
 // ------------------------------------------------------------------ fold --
@@ -334,9 +541,7 @@ __global__ void k_hist(Params pr, int order, unsigned long long* __restrict__ co
     const double eps = src[S_EPS], L = src[S_L];
     atomicAdd(&acc[0], 1ull);
     for (int img = 0; img < 2; ++img) {
-      const int oc = img ? S_IC : S_C;
-      const double d0 = ob[0] - src[oc], d1 = ob[1] - src[oc + 1], d2 = ob[2] - src[oc + 2];
-      const double D = sqrt(d0 * d0 + d1 * d1 + d2 * d2 + eps * eps);
+      const double D = centroid_distance(src + (img ? S_IC : S_C), ob, eps);
       const int cls = D <= 10.0 * L ? 0 : (D > 40.0 * L ? 2 : 1);
       const int base = img ? (band ? 10 : 4) : 1;
       atomicAdd(&acc[base + cls], 1ull);
@@ -344,7 +549,8 @@ __global__ void k_hist(Params pr, int order, unsigned long long* __restrict__ co
     if (band) {
       atomicAdd(&acc[7], 1ull);
     } else {
-      const int nq = order == 0 ? q_order<0>(src, ob) : (order == 1 ? q_order<1>(src, ob) : q_order<2>(src, ob));
+      const int nq = order == 0 ? q_order_fast<0>(src, ob)
+                                : (order == 1 ? q_order_fast<1>(src, ob) : q_order_fast<2>(src, ob));
       atomicAdd(&acc[8], (unsigned long long)(nq * nq));
       atomicAdd(&acc[9], 1ull);
     }
@@ -425,13 +631,33 @@ static int launch_all(const mhs_launch* L, const Params& pr, hipStream_t st) {
   const int nout = pr.n_a * pr.n_b;
   const bool contracted = pr.kind == K_INTERACTION && pr.sb && nout <= 2;
   const bool mixed = pr.flags & F_MIXED;
-  constexpr int NPT = MHS_QC_NPT;
-  constexpr int PPB = 256 / (64 / NPT);
+  // points per lane (two outputs: half as many)
+  constexpr int N32 = MHS_FAR_NPT32, N64 = MHS_FAR_NPT64;
+  constexpr int N32b = N32 > 1 ? N32 / 2 : 1, N64b = N64 > 1 ? N64 / 2 : 1;
   for (int64_t lo = L->obs_lo; lo < L->obs_hi; lo += MHS_ROWS_PER_LAUNCH) {
     const int64_t rows = (L->obs_hi - lo) < MHS_ROWS_PER_LAUNCH ? (L->obs_hi - lo) : MHS_ROWS_PER_LAUNCH;
     Params pc = pr;
     pc.obs_lo = lo;
     const dim3 grid((unsigned)nblk, (unsigned)rows);
+    if (contracted) {
+      // far pass (writes every entry), then the near pass (adds)
+      if (stages & MHS_STAGE_Q) {
+        auto fg = [&](int npt) {   // pairs per block = 256 / (64 / npt)
+          const int64_t per = (int64_t)(4 * npt) * MHS_QC_CT;
+          return dim3((unsigned)((pr.n_cols + per - 1) / per), (unsigned)rows);
+        };
+        if (nout == 1 && mixed) hipLaunchKernelGGL((k_far<float, P, 1, N32>), fg(N32), block, 0, st, pc);
+        else if (nout == 1) hipLaunchKernelGGL((k_far<double, P, 1, N64>), fg(N64), block, 0, st, pc);
+        else if (mixed) hipLaunchKernelGGL((k_far<float, P, 2, N32b>), fg(N32b), block, 0, st, pc);
+        else hipLaunchKernelGGL((k_far<double, P, 2, N64b>), fg(N64b), block, 0, st, pc);
+        if (check(hipGetLastError(), "k_far launch")) return -1;
+      }
+      if (stages & MHS_STAGE_PAIR) {
+        hipLaunchKernelGGL((k_near<P>), grid, block, 0, st, pc);
+        if (check(hipGetLastError(), "k_near launch")) return -1;
+      }
+      continue;
+    }
     if (stages & MHS_STAGE_PAIR) {
       if (pr.kind == K_DISP) hipLaunchKernelGGL((k_pair<P, 1>), grid, block, 0, st, pc);
       else if (pr.kind == K_EIGEN) hipLaunchKernelGGL((k_pair<P, 2>), grid, block, 0, st, pc);
@@ -439,14 +665,7 @@ static int launch_all(const mhs_launch* L, const Params& pr, hipStream_t st) {
       if (check(hipGetLastError(), "k_pair launch")) return -1;
     }
     if (pr.kind == K_EIGEN || !(stages & MHS_STAGE_Q)) continue;
-    if (contracted) {
-      const dim3 qg((unsigned)((pr.n_cols + PPB - 1) / PPB), (unsigned)rows);
-      if (nout == 1 && mixed) hipLaunchKernelGGL((k_qc<float, P, 1, NPT>), qg, block, 0, st, pc);
-      else if (nout == 1) hipLaunchKernelGGL((k_qc<double, P, 1, NPT>), qg, block, 0, st, pc);
-      else if (mixed) hipLaunchKernelGGL((k_qc<float, P, 2, NPT>), qg, block, 0, st, pc);
-      else hipLaunchKernelGGL((k_qc<double, P, 2, NPT>), qg, block, 0, st, pc);
-      if (check(hipGetLastError(), "k_qc launch")) return -1;
-    } else {
+    {
       if (stress) hipLaunchKernelGGL((k_q<P, true>), grid, block, 0, st, pc);
       else hipLaunchKernelGGL((k_q<P, false>), grid, block, 0, st, pc);
       if (check(hipGetLastError(), "k_q launch")) return -1;

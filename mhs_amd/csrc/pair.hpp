@@ -189,6 +189,19 @@ MHS_INLINE int q_order(const double* __restrict__ src, const double o[3]) {
   return (int)fmin(fmax(nq, 8.0), 192.0);
 }
 
+// The same order, settled without the exact distance when a bounding-sphere
+// lower bound on it already puts 10 L / sqrt(delta^2 + eps^2) below the floor
+// (with a relative margin far above rounding): every such pair gets 8.
+template <int P>
+MHS_INLINE int q_order_fast(const double* __restrict__ src, const double o[3]) {
+  const double d0 = o[0] - src[S_IC], d1 = o[1] - src[S_IC + 1], d2 = o[2] - src[S_IC + 2];
+  const double lb = fmax(sqrt(d0 * d0 + d1 * d1 + d2 * d2) - src[S_RC], 0.0);
+  const double eps = src[S_EPS];
+  const double x = 10.0 * src[S_L] / sqrt(lb * lb + eps * eps);
+  if (x <= (8.0 - P) * (1.0 - 1e-9)) return 8;
+  return q_order<P>(src, o);
+}
+
 // Q-family sum for node k (image.py:315-377): accumulates DDG (stress) or DG.
 template <int P, bool STRESS>
 MHS_INLINE void q_family(const Params& pr, const double* __restrict__ src, int k, const double o[3],
@@ -225,9 +238,9 @@ MHS_INLINE void q_family(const Params& pr, const double* __restrict__ src, int k
 // phi_m with the mirror normal nm and the mirror slip sign.
 MHS_INLINE void emit_stress(const Params& pr, int64_t o, int64_t col, int64_t s, const double* H,
                             double phi_d, double phi_m, const double* nd, const double* nm,
-                            bool add) {
+                            bool add, bool add_eig = false) {
   double E[27];
-  const bool eig = !add && (pr.kind == K_EIGEN || (pr.flags & (F_SUBTRACT | F_PARTS)));
+  const bool eig = (!add || add_eig) && (pr.kind == K_EIGEN || (pr.flags & (F_SUBTRACT | F_PARTS)));
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -334,6 +347,69 @@ MHS_INLINE void pair_item(const Params& pr, int64_t o, int64_t col) {
     }
     emit_stress(pr, o, col, s, H, phi_d, phi_m, src + S_N, src + S_IN, false);
   }
+}
+
+// Near part of one (observer row o, element column col) of a contracted
+// interaction, added after the far pass: the closed-form direct term and its
+// eigenstress when the pair is near (sqrt(|x - c|^2 + eps^2) <= 10 L), and on
+// the image side either the folded mirror element (fold band, with its
+// eigenstress) or the closed-form R-family when the mirrored triangle is
+// near.  Everything far is the far pass's.
+template <int P>
+MHS_INLINE void pair_near_item(const Params& pr, int64_t o, int64_t col) {
+  constexpr int K = (P + 1) * (P + 2) / 2;
+  const int64_t s = col / K;
+  const int k = (int)(col - s * K);
+  const double* src = pr.src + s * MHS_SRC_STRIDE;
+  const double ob[3] = {pr.obs[3 * o], pr.obs[3 * o + 1], pr.obs[3 * o + 2]};
+  const double eps = src[S_EPS];
+  const bool band = in_fold_band(src, ob, pr.flags);
+  const bool dnear = centroid_distance(src + S_C, ob, eps) <= 10.0 * src[S_L];
+  const bool rnear = !band && centroid_distance(src + S_IC, ob, eps) <= 10.0 * src[S_IL];
+  if (!dnear && !band && !rnear) return;
+  const Mat mt{pr.mu, pr.lam, pr.nu};
+  double H[27];
+#pragma unroll
+  for (int t = 0; t < 27; ++t) H[t] = 0.0;
+  double phi_d = 0.0, phi_m = 0.0;
+  double W[6][21];
+  double X1, X2;
+  if (dnear) {
+    Frame f;
+    load_frame(src, false, f);
+    const double z = weighted_tables<SpecH<P>, P>(f, src + S_V, k, ob, eps, pr.gl_x, pr.gl_w, W, X1, X2);
+    phi_d = direct_E(W, eps);
+    direct_H(f, W, z, eps, mt, H);
+  }
+  if (band || rnear) {
+    Frame g;
+    load_frame(src, true, g);
+    double vi[9];
+    reflected_vertices(src, vi);
+    if (band) {
+      const double z = weighted_tables<SpecH<P>, P>(g, vi, k, ob, eps, pr.gl_x, pr.gl_w, W, X1, X2);
+      phi_m = direct_E(W, eps);
+      double Hm[27];
+      direct_H(g, W, z, eps, mt, Hm);
+#pragma unroll
+      for (int t = 0; t < 27; ++t) H[t] += ((t % 3) == 2 ? Hm[t] : -Hm[t]);
+    } else {
+      const double zp = weighted_tables<SpecI<P>, P>(g, vi, k, ob, eps, pr.gl_x, pr.gl_w, W, X1, X2);
+      const double al[3] = {-g.e1[0], -g.e1[1], -g.e1[2]};
+      const double be[3] = {-g.e2[0], -g.e2[1], -g.e2[2]};
+      const double ga[3] = {zp * g.n[0], zp * g.n[1], zp * g.n[2]};
+      double DDG[81];
+#pragma unroll
+      for (int t = 0; t < 81; ++t) DDG[t] = 0.0;
+      r_ddg(W[2], W[3], W[4], W[5], al, be, ga, ob[2], pr.coef, DDG);
+      const double nrm[3] = {src[S_N], src[S_N + 1], src[S_N + 2]};
+      double Hi[27];
+      contract_H(DDG, nrm, pr.mu, pr.lam, Hi);
+#pragma unroll
+      for (int t = 0; t < 27; ++t) H[t] += Hi[t];
+    }
+  }
+  emit_stress(pr, o, col, s, H, phi_d, phi_m, src + S_N, src + S_IN, true, true);
 }
 
 // One (observer row, element column) of the Q-family pass (added).

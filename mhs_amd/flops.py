@@ -33,6 +33,20 @@ at order P0 in the stress family (the interaction workload):
   ~2.6e3; near image ~5.2e3; direct lift + contraction + rotation ~3.0e3;
   R-family lift + records + contraction ~6.0e3; epilogue 2e2.
 
+The contracted interaction (one or two outputs; k_far + k_near) evaluates
+the far field pointwise instead (csrc/kernels.hip):
+
+* per point, FP64 in both modes: barycentric point, weight, offsets 26 and
+  the accumulation 2; then R, 1/R 8 (1/Q 2 more), and the series:
+  direct (d_blocked: 11 shared + 49 per output)            -> 96 per point
+  image R (r_blocked, generated: 71 FMAs, 53 products)     -> 231 per point
+  image Q (q_blocked, generated: 166 FMAs, 66 products)    -> 436 per point
+  of which FP32 in the mixed mode: 68, 203, 408;
+* per pair, FP64: classification, source half F and the block's share of the
+  receiver tables ~150; Q kappa 1,002 FMAs 2,004 (not folded); R kappa 432
+  FMAs 864 (R far); direct coefficients ~120 (direct far);
+* k_near runs the closed forms of the near branches (the constants above).
+
 The near-field constants are averages over the edge regimes; the quadrature
 ones are exact per point.
 """
@@ -52,6 +66,10 @@ FLOP_IMAGE_NEAR = 5200
 FLOP_DIRECT_LIFT = 3000
 FLOP_IMAGE_LIFT = 6000
 FLOP_EPILOGUE = 200
+# contracted interaction, pointwise far field
+FLOP_FAR_D_PT, FLOP_FAR_R_PT, FLOP_FAR_Q_PT = 96, 231, 436
+FLOP_FAR_FP64_PT = 28
+FLOP_FAR_PAIR, FLOP_FAR_KQ, FLOP_FAR_KR, FLOP_FAR_KD = 150, 2004, 864, 120
 
 
 def _tri_dist(obs, v):
@@ -129,4 +147,19 @@ def flops(h: dict) -> dict:
             + (h["pairs"] - h["fold_pairs"]) * FLOP_IMAGE_LIFT)
     q = h["q_points"] * FLOP_Q_PT + h["q_pairs"] * FLOP_Q_PAIR
     q_fp32_mixed = h["q_points"] * (FLOP_Q_PT - FLOP_Q_PT_FP64_IN_MIXED)
-    return {"pair": float(pair), "q": float(q), "q_fp32_in_mixed": float(q_fp32_mixed)}
+    # contracted interaction: far pass and near pass
+    d_pts = h["direct_far12"] * 144 + h["direct_far8"] * 64
+    r_pts = h["image_far12"] * 144 + h["image_far8"] * 64
+    d_far = h["direct_far12"] + h["direct_far8"]
+    r_far = h["image_far12"] + h["image_far8"]
+    far = (d_pts * FLOP_FAR_D_PT + r_pts * FLOP_FAR_R_PT + h["q_points"] * FLOP_FAR_Q_PT
+           + h["pairs"] * FLOP_FAR_PAIR + h["q_pairs"] * FLOP_FAR_KQ + r_far * FLOP_FAR_KR
+           + d_far * FLOP_FAR_KD)
+    far_fp32 = (d_pts * (FLOP_FAR_D_PT - FLOP_FAR_FP64_PT) + r_pts * (FLOP_FAR_R_PT - FLOP_FAR_FP64_PT)
+                + h["q_points"] * (FLOP_FAR_Q_PT - FLOP_FAR_FP64_PT))
+    near = (h["direct_near"] * (FLOP_DIRECT_NEAR + FLOP_DIRECT_LIFT + FLOP_EPILOGUE)
+            + h["image_near"] * (FLOP_IMAGE_NEAR + FLOP_IMAGE_LIFT)
+            + h.get("mirror_near", 0) * FLOP_DIRECT_NEAR + h.get("mirror_far12", 0) * 144 * FLOP_DIRECT_PT
+            + h.get("mirror_far8", 0) * 64 * FLOP_DIRECT_PT + h["fold_pairs"] * FLOP_DIRECT_LIFT)
+    return {"pair": float(pair), "q": float(q), "q_fp32_in_mixed": float(q_fp32_mixed),
+            "far": float(far), "far_fp32_in_mixed": float(far_fp32), "near": float(near)}

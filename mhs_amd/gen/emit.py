@@ -48,6 +48,7 @@ class Emitter:
         self.lines = []
         self.n_coef = 0
         self.coef_src = []          # (kind, list of (pnu, Fraction)) per coefficient
+        self.rec_ks = {}            # table name -> per record {pz: coefficient index}
 
     def coef(self, terms):
         """Register one coefficient sum_t num/den nu^pnu; return its index."""
@@ -70,6 +71,7 @@ def emit_r(em: Emitter, name, records, n_ix, n_slots):
         for pz, pnu, num, den in rec[n_ix + 4]:
             by_pz[pz].append((pnu, Fraction(num, den)))
         ks = {pz: em.coef(t) for pz, t in sorted(by_pz.items())}
+        em.rec_ks.setdefault(name, []).append(ks)
         by_mono[(a, b, c)].append((n, slot, ks))
     # monomial tree: parent drops one factor (z first, then y, then x)
     need = set(by_mono)
@@ -190,54 +192,28 @@ def emit_q(em: Emitter, name, records, n_ix):
     em.w()
 
 
-def emit_q_kappa(em: Emitter, records, n_ix):
-    """Contracted Q-family: per pair, kappa_k = sum over the records of
-    monomial k of C[rec] * G[slot] (G = the pair's output functional on the
-    81 DDG slots); per quadrature point, sum_k kappa_k * monomial_k with
-    monomial_k = d1^a d2^b d3^c R^-n Q^-q.
+SYM = {(0, 0): 0, (1, 1): 1, (2, 2): 2, (0, 1): 3, (1, 0): 3, (0, 2): 4, (2, 0): 4, (1, 2): 5, (2, 1): 5}
 
-    The point sum is emitted register-blocked over NPT points that share each
-    kappa read: the spatial monomials d1^a d2^b d3^c are walked once in an
-    x-major chain (one multiply each), each feeds its radial groups' running
-    sums acc[(n, q)], and the groups are combined by nested Horner in 1/Q then
-    1/R.  kappa is laid out in that walk order."""
-    by_mono = defaultdict(list)
-    for r, rec in enumerate(records):
-        slot = slot_of(rec[:n_ix])
-        by_mono[tuple(rec[n_ix:n_ix + 5])].append((slot, r))
-    monos = list(by_mono)
-    groups = sorted({m[3:] for m in monos})
+
+def emit_blocked_sum(em: Emitter, name, monos, groups, combine):
+    """Register-blocked sum_k kap[o * M + k] * d1^a d2^b d3^c * radial_g for
+    NPT points: the spatial monomials by an x-major product chain (one
+    multiply each), each feeding its radial group's running sum acc[g];
+    ``combine`` emits the per-point radial combination into S.  ``monos`` is
+    [(a, b, c, radial key)] in kappa order (spatial part ascending)."""
+    M = len(monos)
     gidx = {g: i for i, g in enumerate(groups)}
-    # x-major walk over the spatial parts
     dparts = sorted({m[:3] for m in monos})
-    monos.sort(key=lambda m: (m[:3], gidx[m[3:]]))
-    base = em.q_ddg_base
-    ptr = [0]
-    slots, coefs = [], []
-    for m in monos:
-        for slot, r in by_mono[m]:
-            slots.append(slot)
-            coefs.append(base + r)
-        ptr.append(len(slots))
-    kidx = {m: i for i, m in enumerate(monos)}
-    em.w(f"#define MHS_QM {len(monos)}")
-    em.w(f"#define MHS_QREC {len(slots)}")
-    em.w(f"#define MHS_QG {len(groups)}")
-    em.w(f"__constant__ short mhs_qm_ptr[{len(ptr)}] = {{{', '.join(map(str, ptr))}}};")
-    em.w(f"__constant__ short mhs_qm_coef[{len(coefs)}] = {{{', '.join(map(str, coefs))}}};")
-    em.w(f"__constant__ unsigned char mhs_qm_slot[{len(slots)}] = {{{', '.join(map(str, slots))}}};")
-    em.w("// s[o * NPT + j] = sum_k kap[o * MHS_QM + k] * monomial_k(point j), for")
-    em.w("// NOUT kappa sets and NPT points (d1, d2, d3, 1/R, 1/Q per point).")
+    by_d = defaultdict(list)
+    for k, m in enumerate(monos):
+        by_d[m[:3]].append((k, m))
     em.w("template <typename T, int NPT, int NOUT>")
-    em.w("__device__ __forceinline__ void q_blocked(const T* __restrict__ kap, const T* x, const T* y,")
+    em.w(f"__device__ __forceinline__ void {name}(const T* __restrict__ kap, const T* x, const T* y,")
     em.w("    const T* z, const T* ir, const T* iq, T* s)")
     em.w("{")
     em.w(f"  T acc[NOUT][NPT][{len(groups)}];")
     em.w("  T xa[NPT], xy[NPT], d[NPT];")
     init = set()
-    by_d = defaultdict(list)
-    for m in monos:
-        by_d[m[:3]].append(m)
     amax = max(a for a, _, _ in dparts)
     for a in range(amax + 1):
         da = [p for p in dparts if p[0] == a]
@@ -272,10 +248,7 @@ def emit_q_kappa(em: Emitter, records, n_ix):
                     src = None if one else "xy"
                 else:
                     one = False
-                    if c == 1:
-                        expr = "z[j]" if xy_one else "xy[j] * z[j]"
-                    else:
-                        expr = "d[j] * z[j]"
+                    expr = ("z[j]" if xy_one else "xy[j] * z[j]") if c == 1 else "d[j] * z[j]"
                     em.w("#pragma unroll")
                     em.w(f"  for (int j = 0; j < NPT; ++j) d[j] = {expr};")
                     src = "d"
@@ -284,51 +257,140 @@ def emit_q_kappa(em: Emitter, records, n_ix):
                     continue
                 em.w("#pragma unroll")
                 em.w("  for (int o = 0; o < NOUT; ++o) {")
-                for m in ms:
-                    em.w(f"    const T k{kidx[m]} = kap[o * {len(monos)} + {kidx[m]}];")
+                for k, _ in ms:
+                    em.w(f"    const T k{k} = kap[o * {M} + {k}];")
                 em.w("#pragma unroll")
                 em.w("    for (int j = 0; j < NPT; ++j) {")
-                for m in ms:
-                    g = gidx[m[3:]]
-                    k = f"k{kidx[m]}"
+                for k, m in ms:
+                    g = gidx[m[3]]
                     tgt = f"acc[o][j][{g}]"
                     if g not in init:
-                        em.w(f"      {tgt} = {k if one else f'{k} * {src}[j]'};")
+                        em.w(f"      {tgt} = {f'k{k}' if one else f'k{k} * {src}[j]'};")
                     else:
-                        em.w(f"      {tgt} = {f'{tgt} + {k}' if one else f'fma({k}, {src}[j], {tgt})'};")
+                        em.w(f"      {tgt} = {f'{tgt} + k{k}' if one else f'fma(k{k}, {src}[j], {tgt})'};")
                 em.w("    }")
                 em.w("  }")
-                for m in ms:
-                    init.add(gidx[m[3:]])
+                for _, m in ms:
+                    init.add(gidx[m[3]])
     assert len(init) == len(groups)
-    # combine: S = sum_n R^n sum_q Q^q acc[(n, q)], nested Horner
-    ns = sorted({g[0] for g in groups})
-    assert ns == list(range(ns[0], ns[-1] + 1)) and ns[0] == 0
-    qmins = {n: min(q for nn, q in groups if nn == n) for n in ns}
     em.w("#pragma unroll")
     em.w("  for (int j = 0; j < NPT; ++j) {")
-    em.w("    const T q1 = iq[j];")
-    maxpow = max(qmins.values())
-    for k in range(2, maxpow + 1):
-        em.w(f"    const T q{k} = q{k - 1} * iq[j];")
-    em.w("#pragma unroll")
-    em.w("    for (int o = 0; o < NOUT; ++o) {")
-    em.w("      T S, t;")
-    for n in reversed(ns):
-        qs = sorted(q for nn, q in groups if nn == n)
-        assert qs == list(range(qs[0], qs[-1] + 1))
-        em.w(f"      t = acc[o][j][{gidx[(n, qs[-1])]}];")
-        for q in reversed(qs[:-1]):
-            em.w(f"      t = fma(t, q1, acc[o][j][{gidx[(n, q)]}]);")
-        if n == ns[-1]:
-            em.w(f"      S = t * q{qmins[n]};")
-        else:
-            em.w(f"      S = fma(S, ir[j], t * q{qmins[n]});")
-    em.w("      s[o * NPT + j] = S;")
-    em.w("    }")
+    combine(em, gidx)
     em.w("  }")
     em.w("}")
     em.w()
+
+
+def _walk_order(monos):
+    """kappa order of the blocked sum: spatial part (x-major), then radial."""
+    return sorted(monos, key=lambda m: (m[:3], m[3]))
+
+
+def _csr(em, prefix, keyed, n_keys, cols):
+    """CSR over keys k * 6 + v of entries (key, *cols) (sorted by key)."""
+    keyed.sort()
+    ptr = [0] * (n_keys + 1)
+    for e in keyed:
+        ptr[e[0] + 1] += 1
+    for t in range(n_keys):
+        ptr[t + 1] += ptr[t]
+    em.w(f"__constant__ short {prefix}_ptr[{len(ptr)}] = {{{', '.join(map(str, ptr))}}};")
+    for ci, (cname, ctype) in enumerate(cols):
+        vals = ", ".join(str(e[ci + 1]) for e in keyed)
+        em.w(f"__constant__ {ctype} {prefix}_{cname}[{len(keyed)}] = {{{vals}}};")
+
+
+def emit_q_kappa(em: Emitter, records, n_ix):
+    """Contracted Q-family: per pair, kappa_k = sum over the records of
+    monomial k of C[rec] * G[slot] (G = the pair's output functional on the
+    81 DDG slots, G[i, j, p, m] = E[i][p] F[j][m] with E the receiver and F
+    the source half, both symmetric); factored as kappa_k = sum_v F_v B[k][v]
+    with the receiver table B[k][v] = sum over the records of k whose (j, m)
+    is the symmetric index v of C[rec] E[i][p].  Per quadrature point
+    sum_k kappa_k d1^a d2^b d3^c R^-n Q^-q, register-blocked; the radial
+    groups (n, q) combine by nested Horner in 1/Q then 1/R."""
+    by_mono = defaultdict(list)
+    for r, rec in enumerate(records):
+        by_mono[tuple(rec[n_ix:n_ix + 3]) + (tuple(rec[n_ix + 3:n_ix + 5]),)].append((slot_of(rec[:n_ix]), r))
+    monos = _walk_order(list(by_mono))
+    groups = sorted({m[3] for m in monos})
+    kidx = {m: i for i, m in enumerate(monos)}
+    base = em.q_ddg_base
+    keyed = []
+    for m in monos:
+        for slot, r in by_mono[m]:
+            i, j, p, mm = slot // 27, (slot // 9) % 3, (slot // 3) % 3, slot % 3
+            keyed.append((kidx[m] * 6 + SYM[(j, mm)], base + r, i * 3 + p))
+    em.w(f"#define MHS_QM {len(monos)}")
+    em.w(f"#define MHS_QREC {len(keyed)}")
+    em.w("// Q B-table CSR over (monomial k, symmetric source index v): record")
+    em.w("// coefficient index and receiver index i * 3 + p.")
+    _csr(em, "mhs_qb", keyed, len(monos) * 6, [("coef", "short"), ("ip", "unsigned char")])
+    ns = sorted({g[0] for g in groups})
+    assert ns == list(range(ns[0], ns[-1] + 1)) and ns[0] == 0
+    qmins = {n: min(q for nn, q in groups if nn == n) for n in ns}
+
+    def combine(em, gidx):
+        em.w("    const T q1 = iq[j];")
+        for k in range(2, max(qmins.values()) + 1):
+            em.w(f"    const T q{k} = q{k - 1} * iq[j];")
+        em.w("#pragma unroll")
+        em.w("    for (int o = 0; o < NOUT; ++o) {")
+        em.w("      T S, t;")
+        for n in reversed(ns):
+            qs = sorted(q for nn, q in groups if nn == n)
+            assert qs == list(range(qs[0], qs[-1] + 1))
+            em.w(f"      t = acc[o][j][{gidx[(n, qs[-1])]}];")
+            for q in reversed(qs[:-1]):
+                em.w(f"      t = fma(t, q1, acc[o][j][{gidx[(n, q)]}]);")
+            if n == ns[-1]:
+                em.w(f"      S = t * q{qmins[n]};")
+            else:
+                em.w(f"      S = fma(S, ir[j], t * q{qmins[n]});")
+        em.w("      s[o * NPT + j] = S;")
+        em.w("    }")
+
+    emit_blocked_sum(em, "q_blocked", monos, groups, combine)
+
+
+def emit_r_kappa(em: Emitter, records, ks_list, n_ix):
+    """Contracted R-family, far field: the same factoring as the Q-family with
+    depth-dependent record coefficients C_rec(z) = sum_pz C[rec, pz] z^pz (z
+    the receiver's x3, fixed per block).  Per quadrature point on the mirrored
+    triangle sum_k kappa_k d1^a d2^b d3^c R^-n, n odd, combined by Horner in
+    1/R^2."""
+    by_mono = defaultdict(list)
+    for rec, ks in zip(records, ks_list):
+        by_mono[tuple(rec[n_ix:n_ix + 3]) + (rec[n_ix + 3],)].append((slot_of(rec[:n_ix]), ks))
+    monos = _walk_order(list(by_mono))
+    groups = sorted({m[3] for m in monos})
+    assert groups == list(range(groups[0], groups[-1] + 1, 2))
+    kidx = {m: i for i, m in enumerate(monos)}
+    keyed = []
+    for m in monos:
+        for slot, ks in by_mono[m]:
+            i, j, p, mm = slot // 27, (slot // 9) % 3, (slot // 3) % 3, slot % 3
+            for pz, ci in ks.items():
+                keyed.append((kidx[m] * 6 + SYM[(j, mm)], ci, pz, i * 3 + p))
+    em.w(f"#define MHS_RM {len(monos)}")
+    em.w(f"#define MHS_RREC {len(keyed)}")
+    em.w("// R B-table CSR over (monomial k, symmetric source index v): coefficient")
+    em.w("// index, power of the receiver depth, receiver index i * 3 + p.")
+    _csr(em, "mhs_rb", keyed, len(monos) * 6, [("coef", "short"), ("pz", "unsigned char"),
+                                                 ("ip", "unsigned char")])
+
+    def combine(em, gidx):
+        em.w("    const T i2 = ir[j] * ir[j];")
+        em.w(f"    const T i{groups[0]} = {' * '.join(['ir[j]'] * groups[0])};")
+        em.w("#pragma unroll")
+        em.w("    for (int o = 0; o < NOUT; ++o) {")
+        em.w(f"      T S = acc[o][j][{gidx[groups[-1]]}];")
+        for n in reversed(groups[:-1]):
+            em.w(f"      S = fma(S, i2, acc[o][j][{gidx[n]}]);")
+        em.w(f"      s[o * NPT + j] = S * i{groups[0]};")
+        em.w("    }")
+
+    emit_blocked_sum(em, "r_blocked", monos, groups, combine)
 
 
 def main():
@@ -389,6 +451,7 @@ def main():
     kap = Emitter()
     kap.q_ddg_base = offs["q_ddg"]
     emit_q_kappa(kap, t["q_ddg"], 4)
+    emit_r_kappa(kap, t["r_ddg"], body.rec_ks["r_ddg"], 4)
     em.lines += kap.lines
     OUT.write_text("\n".join(em.lines) + "\n")
     print(f"wrote {OUT} ({len(em.lines)} lines, {n_coef} coefficients)")

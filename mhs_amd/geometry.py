@@ -24,7 +24,7 @@ S_V, S_EPS = 0, 9
 S_E1, S_E2, S_N, S_C, S_P, S_AREA, S_L = 10, 13, 16, 19, 22, 28, 29
 S_IE1, S_IE2, S_IN, S_IC = 30, 33, 36, 39
 S_ABC, S_CREASE, S_CREASE_R, S_DIAM, S_STRADDLE = 42, 51, 53, 54, 55
-S_IP, S_IABC, S_IAREA, S_IL = 56, 62, 71, 72
+S_IP, S_IABC, S_IAREA, S_IL, S_RC = 56, 62, 71, 72, 73
 
 REFLECT = np.array([1.0, 1.0, -1.0])
 
@@ -102,6 +102,8 @@ def source_records(tris: np.ndarray, eps_arr: np.ndarray) -> np.ndarray:
     diam = np.max(np.stack([np.linalg.norm(tri[:, i] - tri[:, j], axis=1)
                             for i in range(3) for j in range(3)], axis=1), axis=1)
     rec[:, S_DIAM] = diam
+    # bounding radius about the centroid (the same for the mirrored triangle)
+    rec[:, S_RC] = np.linalg.norm(tri - c[:, None, :], axis=2).max(axis=1)
     return rec
 
 
