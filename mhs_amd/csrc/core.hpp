@@ -8,17 +8,19 @@
 #include <stdint.h>
 
 #define MHS_INLINE __host__ __device__ __forceinline__
-// The three largest blocks are real device calls, not inlined: fully inlined
-// into one pair kernel, hipcc (ROCm 7.2, -O2/-O3) produced lane-dependent
-// wrong results (NaN in one lane whenever the lanes of a wave took different
-// near/far/fold branches), while -O1, single-lane launches and any one of
-// these three as a call were correct.
+// The edge integrals and the full direct lift are real device calls, not
+// inlined: with them and the near-field moments all inlined into one pair
+// kernel, hipcc (ROCm 7.2, -O2/-O3) produced lane-dependent wrong results
+// (NaN in one lane whenever the lanes of a wave took different near/far/fold
+// branches), while -O1, single-lane launches and any one of the three as a
+// call were correct.  The near-field moments are inlined (their 216-entry
+// table would otherwise live in scratch memory).
 #define MHS_CALL __host__ __device__ __attribute__((noinline))
 #ifndef MHS_EDGE_FN
 #define MHS_EDGE_FN MHS_CALL
 #endif
 #ifndef MHS_NEAR_FN
-#define MHS_NEAR_FN MHS_CALL
+#define MHS_NEAR_FN MHS_INLINE
 #endif
 #ifndef MHS_DIRECT_FN
 #define MHS_DIRECT_FN MHS_CALL
@@ -747,6 +749,103 @@ MHS_DIRECT_FN void direct_H(const Frame& f, const double W[6][21], double z, dou
 #pragma unroll
       for (int J = 0; J < 3; ++J)
         H[(M_ * 3 + L) * 3 + J] = R[M_][0] * A2[0][L][J] + R[M_][1] * A2[1][L][J] + R[M_][2] * A2[2][L][J];
+}
+
+// Receiver and source halves of a contracted output, symmetric 3 x 3 in the
+// order 00 11 22 01 02 12 (global frame):
+//   E = mu (rb nr^T + nr rb^T) + lam (rb . nr) I   (traction along rb on the
+//       plane of normal nr, as a functional of strain);
+//   F = mu (sb n^T + n sb^T) + lam (sb . n) I      (Hooke of slip sb on the
+//       source plane of normal n).
+MHS_INLINE void half_tensor(const double* u, const double* n, double mu, double lam, double A[6]) {
+  const double un = u[0] * n[0] + u[1] * n[1] + u[2] * n[2];
+  A[0] = 2.0 * mu * u[0] * n[0] + lam * un;
+  A[1] = 2.0 * mu * u[1] * n[1] + lam * un;
+  A[2] = 2.0 * mu * u[2] * n[2] + lam * un;
+  A[3] = mu * (u[0] * n[1] + n[0] * u[1]);
+  A[4] = mu * (u[0] * n[2] + n[0] * u[2]);
+  A[5] = mu * (u[1] * n[2] + n[1] * u[2]);
+}
+
+// d.A.d for the local offset d = (-xi1, -xi2, z) of the frame f as a
+// polynomial in (xi1, xi2), coefficients in table order tix(u, v).
+MHS_INLINE void local_quadratic(const Frame& f, const double A[6], double z, double p[6]) {
+  const double M[3][3] = {{A[0], A[3], A[4]}, {A[3], A[1], A[5]}, {A[4], A[5], A[2]}};
+  const double* Q[3] = {f.e1, f.e2, f.n};
+  double L[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = i; j < 3; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+        s += Q[i][a] * (M[a][0] * Q[j][0] + M[a][1] * Q[j][1] + M[a][2] * Q[j][2]);
+      L[i][j] = s;
+    }
+  p[tix(0, 0)] = z * z * L[2][2];
+  p[tix(1, 0)] = -2.0 * z * L[0][2];
+  p[tix(0, 1)] = -2.0 * z * L[1][2];
+  p[tix(2, 0)] = L[0][0];
+  p[tix(1, 1)] = 2.0 * L[0][1];
+  p[tix(0, 2)] = L[1][1];
+}
+
+// The direct (Kelvin-Cortez) total stress of node k contracted with a
+// receiver half E and a source half F -- rb.H.nr for slip sb -- from the
+// weighted tables: the pointwise kernel
+//   -C1 [A3/R^3 + (A5c c + A5a a + A5b b + A5e)/R^5 + (15 a b + A7c c)/R^7]
+// with a = d.E.d, b = d.F.d, c = d.sym(EF).d (the contraction of
+// kernels.py:182-213's H with rb, nr and sb), each a quadratic in the in-plane
+// offset, integrated against W5 (degree <= 2) and W7 (degree <= 4).  With
+// eig, minus the eigenstress term E_blob * rb.F.nr (assemble.py:123-149).
+MHS_INLINE double direct_contract(const Frame& f, const double W[6][21], double z, double eps,
+                                  const Mat& mt, const double E[6], const double F[6], bool eig,
+                                  double rFn) {
+  const double nu = mt.nu;
+  const double C1 = 1.0 / (16.0 * 3.14159265358979323846 * mt.mu * (1.0 - nu));
+  const double c34 = 3.0 - 4.0 * nu;
+  const double Em[3][3] = {{E[0], E[3], E[4]}, {E[3], E[1], E[5]}, {E[4], E[5], E[2]}};
+  const double Fm[3][3] = {{F[0], F[3], F[4]}, {F[3], F[1], F[5]}, {F[4], F[5], F[2]}};
+  double EF = 0.0, Ms[6];
+  double MM[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      MM[i][j] = Em[i][0] * Fm[0][j] + Em[i][1] * Fm[1][j] + Em[i][2] * Fm[2][j];
+      EF += Em[i][j] * Fm[i][j];
+    }
+  Ms[0] = MM[0][0]; Ms[1] = MM[1][1]; Ms[2] = MM[2][2];
+  Ms[3] = 0.5 * (MM[0][1] + MM[1][0]); Ms[4] = 0.5 * (MM[0][2] + MM[2][0]); Ms[5] = 0.5 * (MM[1][2] + MM[2][1]);
+  const double trE = E[0] + E[1] + E[2], trF = F[0] + F[1] + F[2];
+  double pa[6], pb[6], pc[6];
+  local_quadratic(f, E, z, pa);
+  local_quadratic(f, F, z, pb);
+  local_quadratic(f, Ms, z, pc);
+  const double e2 = eps * eps;
+  const double A3 = (1.0 - c34) * EF + trE * trF;
+  const double A5c = 3.0 * c34 - 9.0, A5a = -3.0 * trF, A5b = -3.0 * trE;
+  const double A5e = -6.0 * (1.0 - nu) * e2 * EF;
+  const double A7c = 30.0 * (1.0 - nu) * e2;
+  const double* W3 = W[2];
+  const double* W5 = W[3];
+  const double* W7 = W[4];
+  double S = A3 * W3[0] + A5e * W5[0];
+  constexpr int U[6] = {0, 1, 0, 2, 1, 0}, V[6] = {0, 0, 1, 0, 1, 2};   // exponents of tix 0..5
+#pragma unroll
+  for (int t = 0; t < 6; ++t) {
+    S = fma(A5c * pc[t] + A5a * pa[t] + A5b * pb[t], W5[t], S);
+    S = fma(A7c * pc[t], W7[t], S);
+  }
+#pragma unroll
+  for (int t1 = 0; t1 < 6; ++t1)
+#pragma unroll
+    for (int t2 = 0; t2 < 6; ++t2)
+      S = fma(15.0 * pa[t1] * pb[t2], W7[tix(U[t1] + U[t2], V[t1] + V[t2])], S);
+  double out = -C1 * S;
+  if (eig) out -= (15.0 * e2 * e2 / (8.0 * 3.14159265358979323846)) * W7[0] * rFn;
+  return out;
 }
 
 MHS_INLINE double direct_E(const double W[6][21], double eps) {

@@ -239,13 +239,16 @@ MHS_INLINE void surface_load(const Params& pr, const double* __restrict__ src, i
   double W[6][21];
   double X1, X2;
   const double z = weighted_tables<SpecH<P>, P>(f, src + S_V, k, o, eps, pr.gl_x, pr.gl_w, W, X1, X2);
-  double H[27];
-  direct_H(f, W, z, eps, mt, H);
-  const double phi = direct_E(W, eps);
-  const double* n = src + S_N;
+  // sigma_zz (receiver half of e_z on the plane z = 0) per unit slip e_m,
+  // minus the eigenstress E C_zz, by the contracted lift
+  const double ez[3] = {0.0, 0.0, 1.0};
+  double E[6];
+  half_tensor(ez, ez, pr.mu, pr.lam, E);
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
-    const double ezz = pr.lam * n[m] + (m == 2 ? 2.0 * pr.mu * n[2] : 0.0);
-    s[m] = 2.0 * (H[(2 * 3 + 2) * 3 + m] - phi * ezz);
+    const double em[3] = {m == 0 ? 1.0 : 0.0, m == 1 ? 1.0 : 0.0, m == 2 ? 1.0 : 0.0};
+    double Fm[6];
+    half_tensor(em, src + S_N, pr.mu, pr.lam, Fm);
+    s[m] = 2.0 * direct_contract(f, W, z, eps, mt, E, Fm, true, Fm[2]);
   }
 }

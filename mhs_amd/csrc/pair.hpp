@@ -368,18 +368,41 @@ MHS_INLINE void pair_near_item(const Params& pr, int64_t o, int64_t col) {
   const bool rnear = !band && centroid_distance(src + S_IC, ob, eps) <= 10.0 * src[S_IL];
   if (!dnear && !band && !rnear) return;
   const Mat mt{pr.mu, pr.lam, pr.nu};
-  double H[27];
-#pragma unroll
-  for (int t = 0; t < 27; ++t) H[t] = 0.0;
-  double phi_d = 0.0, phi_m = 0.0;
+  const bool sub = (pr.flags & F_SUBTRACT) != 0;
+  const int n_a = pr.n_a, n_b = pr.n_b;     // n_a * n_b <= 2
+  const double* nr = pr.nrm + 3 * o;
+  // receiver halves E_a, source halves F_b (element) and G_b (folded mirror:
+  // slip (-sx, -sy, sz) on the mirror normal, fold.py:370-380)
+  double Ea[2][6], Fb[2][6], Gb[2][6], rF[2][2], rG[2][2];
+  for (int a = 0; a < n_a; ++a) half_tensor(pr.rb + ((int64_t)o * n_a + a) * 3, nr, pr.mu, pr.lam, Ea[a]);
+  for (int b = 0; b < n_b; ++b) {
+    const double* sb = pr.sb + (s * n_b + b) * 3;
+    half_tensor(sb, src + S_N, pr.mu, pr.lam, Fb[b]);
+    const double sm[3] = {-sb[0], -sb[1], sb[2]};
+    half_tensor(sm, src + S_IN, pr.mu, pr.lam, Gb[b]);
+    for (int a = 0; a < n_a; ++a) {
+      const double* rb = pr.rb + ((int64_t)o * n_a + a) * 3;
+      double f = 0.0, g = 0.0;
+      const int ix[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          f += rb[i] * Fb[b][ix[i][j]] * nr[j];
+          g += rb[i] * Gb[b][ix[i][j]] * nr[j];
+        }
+      rF[a][b] = f;
+      rG[a][b] = g;
+    }
+  }
+  double acc[2] = {0.0, 0.0};
   double W[6][21];
   double X1, X2;
   if (dnear) {
     Frame f;
     load_frame(src, false, f);
     const double z = weighted_tables<SpecH<P>, P>(f, src + S_V, k, ob, eps, pr.gl_x, pr.gl_w, W, X1, X2);
-    phi_d = direct_E(W, eps);
-    direct_H(f, W, z, eps, mt, H);
+    for (int a = 0; a < n_a; ++a)
+      for (int b = 0; b < n_b; ++b)
+        acc[a * n_b + b] += direct_contract(f, W, z, eps, mt, Ea[a], Fb[b], sub, rF[a][b]);
   }
   if (band || rnear) {
     Frame g;
@@ -388,11 +411,9 @@ MHS_INLINE void pair_near_item(const Params& pr, int64_t o, int64_t col) {
     reflected_vertices(src, vi);
     if (band) {
       const double z = weighted_tables<SpecH<P>, P>(g, vi, k, ob, eps, pr.gl_x, pr.gl_w, W, X1, X2);
-      phi_m = direct_E(W, eps);
-      double Hm[27];
-      direct_H(g, W, z, eps, mt, Hm);
-#pragma unroll
-      for (int t = 0; t < 27; ++t) H[t] += ((t % 3) == 2 ? Hm[t] : -Hm[t]);
+      for (int a = 0; a < n_a; ++a)
+        for (int b = 0; b < n_b; ++b)
+          acc[a * n_b + b] += direct_contract(g, W, z, eps, mt, Ea[a], Gb[b], sub, rG[a][b]);
     } else {
       const double zp = weighted_tables<SpecI<P>, P>(g, vi, k, ob, eps, pr.gl_x, pr.gl_w, W, X1, X2);
       const double al[3] = {-g.e1[0], -g.e1[1], -g.e1[2]};
@@ -402,14 +423,22 @@ MHS_INLINE void pair_near_item(const Params& pr, int64_t o, int64_t col) {
 #pragma unroll
       for (int t = 0; t < 81; ++t) DDG[t] = 0.0;
       r_ddg(W[2], W[3], W[4], W[5], al, be, ga, ob[2], pr.coef, DDG);
-      const double nrm[3] = {src[S_N], src[S_N + 1], src[S_N + 2]};
-      double Hi[27];
-      contract_H(DDG, nrm, pr.mu, pr.lam, Hi);
+      // rb.H.nr.sb of the R-family = sum over slots E[i][p] F[j][m] DDG[i][j][p][m]
+      const int ix[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
+      for (int a = 0; a < n_a; ++a)
+        for (int b = 0; b < n_b; ++b) {
+          double v = 0.0;
 #pragma unroll
-      for (int t = 0; t < 27; ++t) H[t] += Hi[t];
+          for (int t = 0; t < 81; ++t) {
+            const int i = t / 27, j = (t / 9) % 3, p = (t / 3) % 3, m = t % 3;
+            v = fma(Ea[a][ix[i][p]] * Fb[b][ix[j][m]], DDG[t], v);
+          }
+          acc[a * n_b + b] += v;
+        }
     }
   }
-  emit_stress(pr, o, col, s, H, phi_d, phi_m, src + S_N, src + S_IN, true, true);
+  const int64_t oo = o - pr.out_lo;
+  for (int t = 0; t < n_a * n_b; ++t) pr.out[(oo * pr.n_cols + col) * (n_a * n_b) + t] += acc[t];
 }
 
 // One (observer row, element column) of the Q-family pass (added).
