@@ -7,6 +7,8 @@ src/mhs/kernels/assemble.py are replaced by the HIP kernels (mhs_amd.runtime).
 """
 from __future__ import annotations
 
+import threading
+
 import numpy as np
 
 from . import chunking, runtime
@@ -103,8 +105,17 @@ def _columns(tri, p, layout, shared):
     return sn.ids, sn.n, sn
 
 
+_tls = threading.local()
+
+
 def _finish(job, buf):
-    """Run on the GPU and land the result in the caller's buffer or a new one."""
+    """Run on the GPU and land the result in the caller's buffer or a new one.
+    While a planning capture is open on this thread (:func:`plan`), record the
+    validated job instead and return None."""
+    cap = getattr(_tls, "capture", None)
+    if cap is not None:
+        cap.append(job)
+        return None
     out, out2 = runtime.run(job)
     res = runtime.to_host(out, buf)
     if job.parts:
@@ -261,3 +272,20 @@ def elastic_strain_matrix(obs_pts, tris, material, eps, *, subtract_eigenstress=
     buf = _prepare_out(out, (obs.shape[0], 3, 3, n_dof, 3), obs.shape[0], n_dof, "strain", False)
     return _finish(_job("stress", obs, tri, mu, lam, eps_arr, p, ids, sn, fold,
                         subtract=bool(subtract_eigenstress), strain=True), buf)
+
+
+def plan(fn, *args, **kwargs) -> runtime.Job:
+    """The validated :class:`runtime.Job` that ``fn(*args, **kwargs)`` (one of
+    the public matrix functions) would run, without running it: the same
+    coercion, checks and byte ceiling, for callers that execute jobs
+    themselves (the serving loop)."""
+    _tls.capture = []
+    try:
+        fn(*args, **kwargs)
+        jobs = _tls.capture
+    finally:
+        _tls.capture = None
+    if len(jobs) != 1:
+        raise ValueError(f"{getattr(fn, '__name__', fn)!r} is not a single-job matrix function")
+    return jobs[0]
+

@@ -897,6 +897,24 @@ MHS_DIRECT_FN void direct_U(const Frame& f, const double W[6][21], double z, dou
 
 // Image-family contraction (image.py:107-162): DDG[i][j][p][m] -> H[a][b][k];
 // DG[i][j][m] -> U[i][k]; nrm is the real element's vertex-order normal.
+// Hooke on the receiver side of the traction-operator tensor S[i][p][k]:
+// H[a][b][k] = 2 mu sym(S)[a][b] + lam tr(sym S) delta_ab, per slip k.
+MHS_INLINE void stress_from_S(const double* S, double mu, double lam, double H[27]) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double e[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) e[a][b] = 0.5 * (S[(a * 3 + b) * 3 + k] + S[(b * 3 + a) * 3 + k]);
+    const double tr = e[0][0] + e[1][1] + e[2][2];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) H[(a * 3 + b) * 3 + k] = 2.0 * mu * e[a][b] + (a == b ? lam * tr : 0.0);
+  }
+}
+
 MHS_INLINE void contract_H(const double* DDG, const double* nrm, double mu, double lam, double H[27]) {
   double S[3][3][3];
 #pragma unroll

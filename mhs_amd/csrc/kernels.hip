@@ -63,12 +63,69 @@ __global__ void __launch_bounds__(256) k_pair(Params pr) {
 }
 
 // --------------------------------------------------------------- Q-family --
+// Q-family of the uncontracted forms (disp, stress, traction, interaction
+// with more than two components), moments first: a thread per (observer row,
+// element column); per chunk of monomials the budget-law Gauss points are
+// summed into the chunk's moments w d1^a d2^b d3^c R^-n Q^-q, then the
+// chunk's records are folded into the traction-operator tensor S
+// (image.py:107-162); the stress (or displacement) from S is added.
 template <int P, bool STRESS>
-__global__ void __launch_bounds__(256) k_q(Params pr) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_qm(Params pr) {
+  constexpr int K = (P + 1) * (P + 2) / 2;
+  constexpr int NS = STRESS ? 27 : 9;
+  constexpr int NCH = STRESS ? MHS_QMOM_DDG_CHUNKS : MHS_QMOM_DG_CHUNKS;
+  constexpr int CH = STRESS ? MHS_QMOM_DDG_CHUNK : MHS_QMOM_DG_CHUNK;
   const int64_t o = pr.obs_lo + blockIdx.y;
   const int64_t col = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (col >= pr.n_cols) return;
-  q_item<P, STRESS>(pr, o, col);
+  const int64_t s = col / K;
+  const int k = (int)(col - s * K);
+  const double* src = pr.src + s * MHS_SRC_STRIDE;
+  const double ob[3] = {pr.obs[3 * o], pr.obs[3 * o + 1], pr.obs[3 * o + 2]};
+  if (in_fold_band(src, ob, pr.flags)) return;        // the fold replaces R + Q there
+  const int nq = q_order_fast<P>(src, ob);
+  const double* n = src + S_N;
+  const double* v = src + S_V;
+  const double eps2 = src[S_EPS] * src[S_EPS];
+  const double ar2 = 2.0 * src[S_AREA];
+  const int off = nq * (nq - 1) / 2;
+  const double* gx = pr.gl_x + off;
+  const double* gw = pr.gl_w + off;
+  double S[NS];
+#pragma unroll
+  for (int t = 0; t < NS; ++t) S[t] = 0.0;
+#pragma unroll
+  for (int c = 0; c < NCH; ++c) {
+    double M[CH];
+#pragma unroll
+    for (int t = 0; t < CH; ++t) M[t] = 0.0;
+    for (int i = 0; i < nq; ++i) {
+      const double gi = gx[i], wi = gw[i];
+      for (int j = 0; j < nq; ++j) {
+        const double x1 = gi, x2 = gx[j] * (1.0 - gi);
+        const double w = wi * gw[j] * (1.0 - gi) * ar2;
+        const double l0 = 1.0 - x1 - x2;
+        const double y0 = l0 * v[0] + x1 * v[3] + x2 * v[6];
+        const double y1 = l0 * v[1] + x1 * v[4] + x2 * v[7];
+        const double y2 = l0 * v[2] + x1 * v[5] + x2 * v[8];
+        const double wn = w * shape_at<P>(k, l0, x1, x2);
+        const double d1 = ob[0] - y0, d2 = ob[1] - y1, d3 = ob[2] + y2;
+        const double R = sqrt(d1 * d1 + d2 * d2 + d3 * d3 + eps2);
+        const double ir = 1.0 / R, iq = 1.0 / (R - d3);
+        if (STRESS) qmom_ddg_acc(c, d1, d2, d3, ir, iq, wn, M);
+        else qmom_dg_acc(c, d1, d2, d3, ir, iq, wn, M);
+      }
+    }
+    if (STRESS) qmom_ddg_fold(c, M, pr.coef, n, pr.mu, pr.lam, S);
+    else qmom_dg_fold(c, M, pr.coef, n, pr.mu, pr.lam, S);
+  }
+  if (STRESS) {
+    double H[27];
+    stress_from_S(S, pr.mu, pr.lam, H);
+    emit_stress(pr, o, col, s, H, 0.0, 0.0, n, n, true);
+  } else {
+    emit_disp(pr, o, col, S, true);
+  }
 }
 
 
@@ -666,9 +723,9 @@ static int launch_all(const mhs_launch* L, const Params& pr, hipStream_t st) {
     }
     if (pr.kind == K_EIGEN || !(stages & MHS_STAGE_Q)) continue;
     {
-      if (stress) hipLaunchKernelGGL((k_q<P, true>), grid, block, 0, st, pc);
-      else hipLaunchKernelGGL((k_q<P, false>), grid, block, 0, st, pc);
-      if (check(hipGetLastError(), "k_q launch")) return -1;
+      if (stress) hipLaunchKernelGGL((k_qm<P, true>), grid, block, 0, st, pc);
+      else hipLaunchKernelGGL((k_qm<P, false>), grid, block, 0, st, pc);
+      if (check(hipGetLastError(), "k_qm launch")) return -1;
     }
   }
   if (pr.kind == K_EIGEN) return 0;

@@ -393,6 +393,91 @@ def emit_r_kappa(em: Emitter, records, ks_list, n_ix):
     emit_blocked_sum(em, "r_blocked", monos, groups, combine)
 
 
+def emit_q_moments(em: Emitter, fam, records, n_ix, base, chunk):
+    """Moments-first Q-family for the uncontracted forms: per quadrature point
+    only the monomial moments M_k += w d1^a d2^b d3^c R^-n Q^-q of one chunk
+    of monomials; per pair each chunk's records are then folded into the
+    traction-operator tensor S (image.py:107-162): for a DDG record
+    (i, j, p, m) of value v = C M_k, S[i][p][j] += mu n_m v,
+    S[i][p][m] += mu n_j v and, if j == m, S[i][p][k] += lam n_k v; for a DG
+    record (i, j, m) the same with the p index dropped."""
+    by_mono = defaultdict(list)
+    for r, rec in enumerate(records):
+        by_mono[tuple(rec[n_ix:n_ix + 5])].append((rec[:n_ix], base + r))
+    monos = sorted(by_mono, key=lambda m: (m[:3], m[3:]))
+    chunks = [monos[i:i + chunk] for i in range(0, len(monos), chunk)]
+    em.w(f"#define MHS_QMOM_{fam.upper()}_CHUNKS {len(chunks)}")
+    em.w(f"#define MHS_QMOM_{fam.upper()}_CHUNK {chunk}")
+    for c, ms in enumerate(chunks):
+        # accumulate the chunk's moments
+        em.w(f"__device__ __forceinline__ void qmom_{fam}_acc{c}(double x, double y, double z, double ir,")
+        em.w("    double iq, double w, double* M)")
+        em.w("{")
+        amax = max(m[0] for m in ms); bmax = max(m[1] for m in ms); cmax = max(m[2] for m in ms)
+        nmax = max(m[3] for m in ms); qmax = max(m[4] for m in ms)
+        for nm, var, e in (("x", "x", amax), ("y", "y", bmax), ("z", "z", cmax), ("r", "ir", nmax),
+                           ("q", "iq", qmax)):
+            if e >= 1:
+                em.w(f"  const double {nm}1 = {var};")
+            for k in range(2, e + 1):
+                em.w(f"  const double {nm}{k} = {nm}{k - 1} * {var};")
+        seen_d, seen_r = set(), set()
+        for j, (a, b, cz, n, q) in enumerate(ms):
+            dn = f"D{a}{b}{cz}"
+            if (a, b, cz) not in seen_d:
+                f = [f"x{a}"] * (a > 0) + [f"y{b}"] * (b > 0) + [f"z{cz}"] * (cz > 0)
+                em.w(f"  const double {dn} = {' * '.join(f) if f else '1.0'};")
+                seen_d.add((a, b, cz))
+            rn = f"R{n}_{q}"
+            if (n, q) not in seen_r:
+                f = [f"r{n}"] * (n > 0) + [f"q{q}"] * (q > 0)
+                em.w(f"  const double {rn} = w * {' * '.join(f) if f else '1.0'};")
+                seen_r.add((n, q))
+            em.w(f"  M[{j}] = fma({dn}, {rn}, M[{j}]);")
+        em.w("}")
+        em.w()
+        # fold the chunk's records into S
+        em.w(f"__device__ __forceinline__ void qmom_{fam}_fold{c}(const double* M, const double* __restrict__ C,")
+        em.w("    const double* n, double mu, double lam, double* S)")
+        em.w("{")
+        em.w("  const double mn0 = mu * n[0], mn1 = mu * n[1], mn2 = mu * n[2];")
+        em.w("  const double ln0 = lam * n[0], ln1 = lam * n[1], ln2 = lam * n[2];")
+        for j, m in enumerate(ms):
+            for idx, ci in by_mono[m]:
+                if n_ix == 4:
+                    i, jj, p, mm = idx
+                    base_ix = (i * 3 + p) * 3
+                else:
+                    i, jj, mm = idx
+                    base_ix = i * 3
+                em.w(f"  {{ const double v = C[{ci}] * M[{j}];")
+                em.w(f"    S[{base_ix + jj}] = fma(mn{mm}, v, S[{base_ix + jj}]);")
+                em.w(f"    S[{base_ix + mm}] = fma(mn{jj}, v, S[{base_ix + mm}]);")
+                if jj == mm:
+                    for k in range(3):
+                        em.w(f"    S[{base_ix + k}] = fma(ln{k}, v, S[{base_ix + k}]);")
+                em.w("  }")
+        em.w("}")
+        em.w()
+    # dispatchers
+    for kind in ("acc", "fold"):
+        if kind == "acc":
+            em.w(f"__device__ __forceinline__ void qmom_{fam}_acc(int c, double x, double y, double z, double ir,")
+            em.w("    double iq, double w, double* M)")
+        else:
+            em.w(f"__device__ __forceinline__ void qmom_{fam}_fold(int c, const double* M, const double* __restrict__ C,")
+            em.w("    const double* n, double mu, double lam, double* S)")
+        em.w("{")
+        em.w("  switch (c) {")
+        for c in range(len(chunks)):
+            args = "x, y, z, ir, iq, w, M" if kind == "acc" else "M, C, n, mu, lam, S"
+            em.w(f"    case {c}: qmom_{fam}_{kind}{c}({args}); break;")
+        em.w("    default: break;")
+        em.w("  }")
+        em.w("}")
+        em.w()
+
+
 def main():
     t = json.loads(TABLE.read_text())
     em = Emitter()
@@ -452,6 +537,8 @@ def main():
     kap.q_ddg_base = offs["q_ddg"]
     emit_q_kappa(kap, t["q_ddg"], 4)
     emit_r_kappa(kap, t["r_ddg"], body.rec_ks["r_ddg"], 4)
+    emit_q_moments(kap, "ddg", t["q_ddg"], 4, offs["q_ddg"], 42)
+    emit_q_moments(kap, "dg", t["q_dg"], 3, offs["q_dg"], 35)
     em.lines += kap.lines
     OUT.write_text("\n".join(em.lines) + "\n")
     print(f"wrote {OUT} ({len(em.lines)} lines, {n_coef} coefficients)")
