@@ -84,6 +84,32 @@ def cpu_baseline(n_cols: int, cores: int) -> dict:
                       f"{dt:.1f} s wall on {used} processes"}
 
 
+def _progress(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def served_stream(n_req: int, precision: str, n_cal: int = 48) -> dict:
+    """The served stream: calibrate E[service] on the first ``n_cal`` requests
+    run one at a time, set the Poisson rate to 0.7 / E[service], then serve
+    ``n_req`` requests through the continuous-batching loop (4 slots, results
+    dropped after completion); p50 / p99 latency and pair-evals/s."""
+    from dataclasses import replace
+    from mhs_amd import serve
+    _progress(f"served stream: calibrating on {n_cal} requests")
+    cal = serve.c5_requests(n_req=n_cal, rate=1.0, precision=precision)
+    res = serve.Server(slots=1).run([replace(r, arrival=0.0) for r in cal], keep=False, realtime=False)
+    service = float(np.mean([s.done - s.admitted for s in res if s.error is None]))
+    rate = 0.7 / service
+    _progress(f"served stream: {n_req} requests at {rate:.2f}/s")
+    reqs = serve.c5_requests(n_req=n_req, rate=rate, precision=precision)
+    out = serve.Server(slots=4).run(reqs, keep=False)
+    summ = serve.summary(out)
+    summ.update({"precision": precision, "rate_per_s": rate, "mean_service_s": service,
+                 "calibration": f"first {n_cal} requests of the same seeded stream, one at a time",
+                 "slots": 4})
+    return summ
+
+
 def _job(mixed):
     from mhs_amd import runtime, tdcs, workloads
     from mhs_amd.api import collocation_points
@@ -134,6 +160,8 @@ def main() -> int:
     ap.add_argument("--mode", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--cpu-cols", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=512,
+                    help="requests of the served stream (0 = skip it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the assembly shards by "
@@ -143,10 +171,12 @@ def main() -> int:
 
     from mhs_amd import flops
     mixed = args.mode == "mixed"
+    _progress("timing the other mode")
     prep_o, dt_o, stage_o = _time_mode(not mixed, args.steps, args.warmup)
     ref = prep_o.final.clone()
     hist = prep_o.histogram()
     del prep_o
+    _progress(f"timing {args.mode}")
     prep, dt, stage_ms = _time_mode(mixed, args.steps, args.warmup)
     a, b = (prep.final, ref) if mixed else (ref, prep.final)
     relerr = float(((a - b).abs().max() / b.abs().max()).item())
@@ -192,8 +222,11 @@ def main() -> int:
         "histogram": hist,
         "flop": fl,
     }
+    if args.serve_requests > 0:
+        line["served"] = served_stream(args.serve_requests, args.mode)
     if not args.no_cpu:
         cores = min(os.cpu_count() or 1, 16)
+        _progress(f"CPU baseline on {cores} processes")
         line["cpu_baseline"] = cpu_baseline(args.cpu_cols, cores)
         line["vs_cpu"] = value / line["cpu_baseline"]["value"]
     print(json.dumps(line), flush=True)

@@ -232,25 +232,42 @@ class Prepared:
             fold_list=_ptr(self.fold_list), fold_count=_ptr(self.fold_cnt), fold_err=_ptr(self.fold_err),
             fold_cap=self.fold_cap, stages=stages)
 
+    def chunks(self, max_pairs: int | None = None):
+        """Observer-row ranges that together cover the job, each at most
+        ``max_pairs`` (observer, column) pairs (and, in the continuous layout,
+        at most the element scratch's rows)."""
+        job = self.job
+        step = self.rows
+        if max_pairs is not None:
+            step = max(1, min(step, int(max_pairs) // max(self.n_cols, 1)))
+        return [(lo, min(job.n_obs, lo + step)) for lo in range(0, job.n_obs, step)]
+
+    def launch_rows(self, lo: int, hi: int, stages: int = 0):
+        """Enqueue every pass for observer rows [lo, hi) (at most ``self.rows``
+        of them in the continuous layout); no host synchronisation."""
+        job, lib = self.job, self.dev.lib
+        if self.cont:
+            if hi - lo > self.rows:
+                raise ValueError(f"rows [{lo}, {hi}) exceed the element scratch ({self.rows} rows)")
+            out_p, out2_p = _ptr(self.scr), _ptr(self.scr2)
+        else:
+            row_elems = int(np.prod(self.final.shape[1:]))
+            out_p = C.c_void_p(self.final.data_ptr() + 8 * lo * row_elems)
+            out2_p = (C.c_void_p(self.final2.data_ptr() + 8 * lo * row_elems)
+                      if job.parts else None)
+        L = self._launch_struct(lo, hi, out_p, out2_p, stages)
+        _lib.check(lib.mhs_assemble(C.byref(L), self.handle), lib)
+        if self.cont:
+            self._gather(lo, hi)
+        if job.strain:
+            row_elems = int(np.prod(self.final.shape[1:]))
+            _lib.check(lib.mhs_strain(C.c_void_p(self.final.data_ptr() + 8 * lo * row_elems), hi - lo,
+                                      job.n_dof, job.mu, job.lam, self.handle), lib)
+
     def launch(self, stages: int = 0):
         """Enqueue every pass on the stream (no host synchronisation)."""
-        job, lib = self.job, self.dev.lib
-        for lo in range(0, job.n_obs, self.rows):
-            hi = min(job.n_obs, lo + self.rows)
-            if self.cont:
-                out_p, out2_p = _ptr(self.scr), _ptr(self.scr2)
-            else:
-                row_elems = int(np.prod(self.final.shape[1:]))
-                out_p = C.c_void_p(self.final.data_ptr() + 8 * lo * row_elems)
-                out2_p = (C.c_void_p(self.final2.data_ptr() + 8 * lo * row_elems)
-                          if job.parts else None)
-            L = self._launch_struct(lo, hi, out_p, out2_p, stages)
-            _lib.check(lib.mhs_assemble(C.byref(L), self.handle), lib)
-            if self.cont:
-                self._gather(lo, hi)
-        if job.strain:
-            _lib.check(lib.mhs_strain(_ptr(self.final), job.n_obs, job.n_dof, job.mu, job.lam,
-                                      self.handle), lib)
+        for lo, hi in self.chunks():
+            self.launch_rows(lo, hi, stages)
 
     def _gather(self, lo, hi):
         job, lib = self.job, self.dev.lib
@@ -288,6 +305,26 @@ class Prepared:
             err = int(self.fold_err.cpu()[0])
             if err:
                 raise RuntimeError(f"mhs_amd: fold grid/list overflow (flags {err})")
+
+
+def job_bytes(job: Job) -> int:
+    """Device bytes a :class:`Prepared` of ``job`` holds: outputs, the
+    continuous layout's element scratch, the fold list and the inputs."""
+    out = 8 * int(np.prod(job.out_shape())) * (2 if job.parts else 1)
+    n_cols = job.n_src * job.K
+    scratch = 0
+    rows = job.n_obs
+    if job.ids is not None:
+        shape = list(job.out_shape(n_cols))
+        if job.kind == "interaction":
+            shape[-1] = 3
+        row_bytes = 8 * int(np.prod(shape[1:])) * (2 if job.parts else 1)
+        rows = max(1, min(job.n_obs, ROW_SCRATCH_BYTES // max(row_bytes, 1)))
+        scratch = rows * row_bytes
+    n_strad = int(np.sum(np.asarray(job.tris)[:, :, 2].max(axis=1) > -3.0 * np.asarray(job.eps)))
+    fold = 16 * max(1, rows * n_strad) if job.fold and n_strad else 0
+    inputs = 8 * (job.n_obs * 3 * (2 + job.n_a) + job.n_src * (geometry.SRC_STRIDE + 3 * max(job.n_b, 0)))
+    return int(out + scratch + fold + inputs)
 
 
 def run(job: Job, stream=None, dev: Device | None = None):

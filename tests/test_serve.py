@@ -1,0 +1,126 @@
+"""Host logic of the serving loop (mhs_amd/serve.py) with a CPU stand-in for
+the device: first-come first-served admission by predicted bytes, the slot
+limit, refusal of a request larger than the budget, validation errors as
+results, chunked issue, and the latency summary."""
+import time
+
+import numpy as np
+import pytest
+
+import mhs_amd as M
+from mhs_amd import runtime, serve, workloads
+
+
+class _Event:
+    def __init__(self, t_done):
+        self.t_done = t_done
+
+    def query(self):
+        return time.perf_counter() >= self.t_done
+
+
+class _Handle:
+    def __init__(self, job):
+        self.job = job
+
+    def chunks(self, max_pairs):
+        step = max(1, max_pairs // (self.job.n_src * self.job.K))
+        return [(lo, min(self.job.n_obs, lo + step)) for lo in range(0, self.job.n_obs, step)]
+
+
+class FakeBackend:
+    """Chunks complete 1 ms after issue; records the slots in use."""
+
+    def __init__(self):
+        self.live = set()
+        self.max_live = 0
+        self.issued = []
+
+    def default_budget(self):
+        return 1 << 40
+
+    def prepare(self, job, slot):
+        assert slot not in self.live
+        self.live.add(slot)
+        self.max_live = max(self.max_live, len(self.live))
+        h = _Handle(job)
+        h.slot = slot
+        h.serial = self.n_prepared = getattr(self, "n_prepared", 0) + 1
+        return h
+
+    def issue(self, h, lo, hi):
+        self.issued.append((h.serial, lo, hi))
+        return _Event(time.perf_counter() + 1e-3)
+
+    def finish(self, h):
+        self.live.discard(h.slot)
+        return ("out", h.job.kind), None
+
+
+def _disp_request(n_obs, arrival):
+    obs, tris, eps = workloads.c1_disp()
+    return serve.Request(M.disp_matrix, (obs[:n_obs], tris[:16], M.Material(30.0, 30.0), eps),
+                         arrival=arrival)
+
+
+def test_all_complete_with_slot_limit_and_chunks():
+    be = FakeBackend()
+    reqs = [_disp_request(64 + 8 * i, 0.0) for i in range(7)]
+    srv = serve.Server(slots=3, chunk_pairs=16 * 20, backend=be)
+    out = srv.run(reqs)
+    assert all(s.error is None for s in out)
+    assert be.max_live <= 3
+    assert all(s.out == ("out", "disp") for s in out)
+    # every request's rows are issued exactly once, in chunks of <= 20 rows
+    rows = {}
+    for h, lo, hi in be.issued:
+        assert hi - lo <= 20
+        rows.setdefault(h, []).append((lo, hi))
+    assert sorted(sum(hi - lo for lo, hi in v) for v in rows.values()) == sorted(64 + 8 * i for i in range(7))
+    assert all(s.latency >= 0 and s.admitted >= s.arrival - 1e-9 for s in out)
+    summ = serve.summary(out)
+    assert summ["completed"] == 7 and summ["p99_s"] >= summ["p50_s"] > 0
+
+
+def test_admission_is_fifo_by_bytes():
+    be = FakeBackend()
+    reqs = [_disp_request(200, 0.0), _disp_request(10, 0.0), _disp_request(200, 0.0)]
+    big = runtime.job_bytes(M.api.plan(reqs[0].fn, *reqs[0].args))
+    small = runtime.job_bytes(M.api.plan(reqs[1].fn, *reqs[1].args))
+    # room for one big request at a time: the small one waits behind the first big one
+    srv = serve.Server(slots=4, budget_bytes=big + small // 2, backend=be)
+    out = srv.run(reqs)
+    assert all(s.error is None for s in out)
+    assert out[1].admitted >= out[0].done - 1e-3
+    assert out[2].admitted >= out[1].admitted
+
+
+def test_oversized_and_invalid_requests_are_results():
+    be = FakeBackend()
+    ok = _disp_request(10, 0.0)
+    bad = serve.Request(M.disp_matrix, (np.zeros((4, 3)) + [0, 0, 1.0], ok.args[1], ok.args[2], 0.1))
+    srv = serve.Server(slots=2, budget_bytes=1000, backend=be)
+    out = srv.run([ok, bad])
+    assert out[0].error.startswith("MemoryError")
+    assert out[1].error.startswith("ValueError")
+    assert serve.summary(out)["refused"] == 2
+
+
+def test_realtime_arrivals_are_respected():
+    be = FakeBackend()
+    reqs = [_disp_request(16, 0.0), _disp_request(16, 0.05)]
+    out = serve.Server(slots=2, backend=be).run(reqs)
+    assert out[1].admitted >= 0.05 - 1e-3
+
+
+def test_c5_stream_shapes():
+    reqs = serve.c5_requests(n_req=40, rate=10.0, seed=7)
+    kinds = [r.tag.split()[0] for r in reqs]
+    assert set(kinds) <= {"disp", "traction", "interaction"}
+    arr = np.array([r.arrival for r in reqs])
+    assert np.all(np.diff(arr) > 0)
+    for r in reqs:
+        job = M.api.plan(r.fn, *r.args, **r.kwargs)
+        if job.kind == "interaction":
+            assert job.n_obs <= serve.INTERACTION_MAX_DOFS
+        assert runtime.job_bytes(job) > 0
