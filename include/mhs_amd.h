@@ -23,7 +23,9 @@
 extern "C" {
 #endif
 
-#define MHS_ABI_VERSION 1
+#define MHS_ABI_VERSION 2
+
+#define MHS_GQ_TOTAL 2377760   /* sum of n^2 for n = 1..192 */
 
 /* Output kinds: the five public matrix forms (matrices.py:231-580).
  *   MHS_DISP         (n_obs, 3, n_cols, 3)     disp_matrix
@@ -61,6 +63,10 @@ typedef struct mhs_launch {
   const double* coef;       /* mhs_ncoef() image-kernel coefficients from mhs_fill_material */
   const double* gl_x;       /* Gauss-Legendre nodes on [0, 1] for orders 1..192, order n at n(n-1)/2 */
   const double* gl_w;       /* matching weights */
+  const double* gq;         /* collapsed Gauss rules on the triangle, order n's n^2 points at
+                               (n-1) n (2n-1) / 6, structure of arrays: x1 | x2 | w, each
+                               MHS_GQ_TOTAL long (x1 = u_i, x2 = u_j (1 - u_i),
+                               w = w_i w_j (1 - u_i) of the order-n Gauss-Legendre rule) */
   double* out;              /* output rows [obs_lo, obs_hi) in the kind's layout */
   double* out2;             /* eigenstress part (MHS_PARTS), same layout, or NULL */
   const int64_t* strad;     /* indices of the straddling (fold) sources, n_strad of them */

@@ -10,7 +10,7 @@ import ctypes as C
 import pathlib
 
 LIB_PATH = pathlib.Path(__file__).resolve().parent / "libmhs_amd.so"
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 KIND = {"disp": 0, "stress": 1, "eigen": 2, "traction": 3, "interaction": 4}
 F_FOLD, F_SUBTRACT, F_PARTS, F_MIXED = 1, 2, 4, 8
@@ -27,6 +27,7 @@ class Launch(C.Structure):
         ("mu", C.c_double), ("lam", C.c_double),
         ("obs", C.c_void_p), ("nrm", C.c_void_p), ("rb", C.c_void_p), ("sb", C.c_void_p),
         ("src", C.c_void_p), ("coef", C.c_void_p), ("gl_x", C.c_void_p), ("gl_w", C.c_void_p),
+        ("gq", C.c_void_p),
         ("out", C.c_void_p), ("out2", C.c_void_p),
         ("strad", C.c_void_p), ("n_strad", C.c_int64),
         ("fold_list", C.c_void_p), ("fold_count", C.c_void_p), ("fold_err", C.c_void_p),

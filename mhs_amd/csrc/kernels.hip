@@ -24,11 +24,14 @@
 #ifndef MHS_FAR_NPT64
 #define MHS_FAR_NPT64 2   // quadrature points per lane of the far pass, FP64 points
 #endif
-#ifndef MHS_QC_WAVES
-#define MHS_QC_WAVES 3 // occupancy target of the far pass (waves per SIMD)
+#ifndef MHS_FAR_WAVES32
+#define MHS_FAR_WAVES32 3 // occupancy target of the far pass, FP32 points (waves per SIMD)
+#endif
+#ifndef MHS_FAR_WAVES64
+#define MHS_FAR_WAVES64 2 // occupancy target of the far pass, FP64 points
 #endif
 #ifndef MHS_QC_CT
-#define MHS_QC_CT 8    // column tiles per block of the contracted Q pass
+#define MHS_QC_CT 16   // column tiles per block of the far pass
 #endif
 
 static thread_local char g_err[512];
@@ -188,22 +191,42 @@ __device__ __forceinline__ void d_blocked(const T* __restrict__ kd, const T* x, 
   }
 }
 
-// One quadrature pass of a pair over ng x ng collapsed Gauss points of the
-// source triangle (IMG false: d = x - y) or of its mirror (IMG true:
+// 1/sqrt and 1/x of the point series: the hardware approximations (1 ulp)
+// in FP32, correctly rounded in FP64.
+__device__ __forceinline__ void radii(float R2, float d3, bool q, float& ir, float& iq) {
+  ir = __builtin_amdgcn_rsqf(R2);
+  iq = q ? __builtin_amdgcn_rcpf(R2 * ir - d3) : 0.0f;
+}
+__device__ __forceinline__ void radii(double R2, double d3, bool q, double& ir, double& iq) {
+  const double R = sqrt(R2);
+  ir = 1.0 / R;
+  iq = q ? 1.0 / (R - d3) : 0.0;
+}
+
+// One quadrature pass of a pair over the ng x ng collapsed Gauss points of
+// the source triangle (IMG false: d = x - y) or of its mirror (IMG true:
 // d = x - y*), summing the selected series into acc_d (direct) / acc_i.
-template <typename T, int P, int NOUT, int NPT, bool IMG, bool DIR, bool RF, bool QF>
+// LPP lanes share the points, NPT per lane per chunk of LPP * NPT.  The
+// point comes from the flattened rule table; the offset is
+// d = (x - v0) - x1 (v1 - v0) - x2 (v2 - v0) in FP64, mirrored in z for IMG.
+template <typename T, int P, int NOUT, int NPT, int LPP, bool IMG, bool DIR, bool RF, bool QF>
 __device__ __forceinline__ void far_pass(const Params& pr, const double* __restrict__ src, int k,
                                          const double ob[3], int ng, int li, const T* kd, const T* kr,
                                          const T* kq, double* acc_d, double* acc_i) {
-  constexpr int LPP = 64 / NPT;
+  constexpr int CH = LPP * NPT;
   const double* v = src + S_V;
-  const double eps2 = src[S_EPS] * src[S_EPS];
+  const double sz = IMG ? -1.0 : 1.0;           // z of the (mirrored) vertices
+  const double ov[3] = {ob[0] - v[0], ob[1] - v[1], ob[2] - sz * v[2]};
+  const double e1[3] = {v[3] - v[0], v[4] - v[1], sz * (v[5] - v[2])};
+  const double e2[3] = {v[6] - v[0], v[7] - v[1], sz * (v[8] - v[2])};
+  const T eps2 = (T)(src[S_EPS] * src[S_EPS]);
   const double ar2 = 2.0 * src[S_AREA];
-  const int off = ng * (ng - 1) / 2;
-  const double* gx = pr.gl_x + off;
-  const double* gw = pr.gl_w + off;
+  const size_t toff = (size_t)(ng - 1) * ng * (2 * ng - 1) / 6;
+  const double* qx1 = pr.gq + toff;
+  const double* qx2 = qx1 + MHS_GQ_TOTAL;
+  const double* qw = qx1 + 2 * MHS_GQ_TOTAL;
   const int npts = ng * ng;
-  for (int base = 0; base < npts; base += 64) {
+  for (int base = 0; base < npts; base += CH) {
     // compiler-only fence: keeps the kappa reads inside the chunk instead of
     // hoisting all of them out of the loop into registers
     __asm__ volatile("" ::: "memory");
@@ -214,20 +237,13 @@ __device__ __forceinline__ void far_pass(const Params& pr, const double* __restr
       const int g0 = base + u * LPP + li;
       const bool in = g0 < npts;
       const int g = in ? g0 : 0;
-      const int i = g / ng, j = g - i * ng;
-      const double gi = gx[i];
-      const double x1 = gi, x2 = gx[j] * (1.0 - gi);
-      const double w = in ? gw[i] * gw[j] * (1.0 - gi) * ar2 : 0.0;
-      const double l0 = 1.0 - x1 - x2;
-      const double y0 = l0 * v[0] + x1 * v[3] + x2 * v[6];
-      const double y1 = l0 * v[1] + x1 * v[4] + x2 * v[7];
-      const double y2 = l0 * v[2] + x1 * v[5] + x2 * v[8];
-      wn[u] = w * shape_at<P>(k, l0, x1, x2);
-      const T d1 = (T)(ob[0] - y0), d2 = (T)(ob[1] - y1), d3 = (T)(IMG ? ob[2] + y2 : ob[2] - y2);
-      const T R = sqrt(d1 * d1 + d2 * d2 + d3 * d3 + (T)eps2);
+      const double x1 = qx1[g], x2 = qx2[g];
+      wn[u] = in ? qw[g] * ar2 * shape_at<P>(k, 1.0 - x1 - x2, x1, x2) : 0.0;
+      const T d1 = (T)fma(-x1, e1[0], fma(-x2, e2[0], ov[0]));
+      const T d2 = (T)fma(-x1, e1[1], fma(-x2, e2[1], ov[1]));
+      const T d3 = (T)fma(-x1, e1[2], fma(-x2, e2[2], ov[2]));
       x[u] = d1; y[u] = d2; z[u] = d3;
-      ir[u] = (T)1 / R;
-      iq[u] = QF ? (T)1 / (R - d3) : (T)0;
+      radii(d1 * d1 + d2 * d2 + d3 * d3 + eps2, d3, QF, ir[u], iq[u]);
     }
     if (DIR) {
       d_blocked<T, NPT, NOUT>(kd, x, y, z, ir, sv);
@@ -256,7 +272,8 @@ __device__ __forceinline__ void far_pass(const Params& pr, const double* __restr
 }
 
 template <typename T, int P, int NOUT, int NPT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MHS_QC_WAVES)))
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? MHS_FAR_WAVES32 : MHS_FAR_WAVES64)))
 k_far(Params pr) {
   constexpr int K = (P + 1) * (P + 2) / 2;
   constexpr int LPP = 64 / NPT;
@@ -413,14 +430,24 @@ k_far(Params pr) {
     const T* kd = kd_s[grp];
     const T* kr = kr_s[grp];
     const T* kq = kq_s[grp];
-    if (valid && dfar)
-      far_pass<T, P, NOUT, NPT, false, true, false, false>(pr, src, k, ob, nd, li, kd, kr, kq, acc_d, acc_i);
+    // 12 x 12 rules in chunks of 3 points per lane (16 lanes: 144 = 3 x 48)
+    constexpr int N12 = LPP == 16 ? 3 : NPT;
+    if (valid && dfar) {
+      if (nd == 12)
+        far_pass<T, P, NOUT, N12, LPP, false, true, false, false>(pr, src, k, ob, 12, li, kd, kr, kq, acc_d, acc_i);
+      else
+        far_pass<T, P, NOUT, NPT, LPP, false, true, false, false>(pr, src, k, ob, nd, li, kd, kr, kq, acc_d, acc_i);
+    }
     // the R and Q series run as separate passes even at equal orders: summed
-    // in one pass their registers no longer fit two waves per SIMD
-    if (valid && img && rfar)
-      far_pass<T, P, NOUT, NPT, true, false, true, false>(pr, src, k, ob, ni, li, kd, kr, kq, acc_d, acc_i);
+    // in one pass their registers no longer fit the occupancy target
+    if (valid && img && rfar) {
+      if (ni == 12)
+        far_pass<T, P, NOUT, N12, LPP, true, false, true, false>(pr, src, k, ob, 12, li, kd, kr, kq, acc_d, acc_i);
+      else
+        far_pass<T, P, NOUT, NPT, LPP, true, false, true, false>(pr, src, k, ob, ni, li, kd, kr, kq, acc_d, acc_i);
+    }
     if (valid && img)
-      far_pass<T, P, NOUT, NPT, true, false, false, true>(pr, src, k, ob, nq, li, kd, kr, kq, acc_d, acc_i);
+      far_pass<T, P, NOUT, NPT, LPP, true, false, false, true>(pr, src, k, ob, nq, li, kd, kr, kq, acc_d, acc_i);
     double res[NOUT];
 #pragma unroll
     for (int t = 0; t < NOUT; ++t) {
@@ -767,7 +794,7 @@ int mhs_assemble(const mhs_launch* L, void* stream) {
   if (L->abi_version != MHS_ABI_VERSION) return fail("mhs_assemble: ABI version mismatch");
   if (L->order < 0 || L->order > 2) return fail("mhs_assemble: order must be 0, 1 or 2");
   if (L->kind < K_DISP || L->kind > K_INTERACTION) return fail("mhs_assemble: unknown kind");
-  if (!L->obs || !L->src || !L->coef || !L->gl_x || !L->gl_w || !L->out)
+  if (!L->obs || !L->src || !L->coef || !L->gl_x || !L->gl_w || !L->gq || !L->out)
     return fail("mhs_assemble: null device pointer");
   if ((L->kind == K_TRACTION || L->kind == K_INTERACTION) && !L->nrm)
     return fail("mhs_assemble: traction/interaction need receiver normals");
@@ -783,7 +810,7 @@ int mhs_assemble(const mhs_launch* L, void* stream) {
   if (L->n_cols != L->n_src * K) return fail("mhs_assemble: n_cols must be n_src * K");
   Params pr;
   pr.obs = L->obs; pr.nrm = L->nrm; pr.rb = L->rb; pr.sb = L->sb; pr.src = L->src;
-  pr.coef = L->coef; pr.gl_x = L->gl_x; pr.gl_w = L->gl_w; pr.out = L->out; pr.out2 = L->out2;
+  pr.coef = L->coef; pr.gl_x = L->gl_x; pr.gl_w = L->gl_w; pr.gq = L->gq; pr.out = L->out; pr.out2 = L->out2;
   pr.n_obs = L->n_obs; pr.n_src = L->n_src; pr.n_cols = L->n_cols; pr.obs_lo = L->obs_lo;
   pr.out_lo = L->obs_lo;
   pr.kind = L->kind; pr.flags = L->flags; pr.n_a = L->n_a; pr.n_b = L->n_b;

@@ -18,6 +18,7 @@ struct Params {
   const double* coef;    // MHS_NCOEF image coefficients for (mu, lam)
   const double* gl_x;    // Gauss-Legendre nodes on [0, 1], orders 1..192
   const double* gl_w;
+  const double* gq;      // flattened collapsed Gauss rules: x1 | x2 | w, MHS_GQ_TOTAL each
   double* out;
   double* out2;
   int64_t n_obs, n_src, n_cols;

@@ -107,6 +107,34 @@ def source_records(tris: np.ndarray, eps_arr: np.ndarray) -> np.ndarray:
     return rec
 
 
+GQ_TOTAL = 2377760      # sum of n^2 for n = 1..192
+
+
+@functools.lru_cache(maxsize=1)
+def gauss_triangle_tables():
+    """The collapsed Gauss rules on the reference triangle for orders
+    1..192, flattened: order n's n^2 points at (n-1) n (2n-1) / 6 in three
+    arrays x1 | x2 | w (x1 = u_i, x2 = u_j (1 - u_i), w = w_i w_j (1 - u_i),
+    point g = i n + j), the same points and order as the kernels' nested loops
+    (moments.py:427-498, image.py:315-377)."""
+    gx, gw = gauss_tables()
+    out = np.empty((3, GQ_TOTAL))
+    off = 0
+    for n in range(1, defaults.IMAGE_Q_GAUSS_MAX + 1):
+        o1 = n * (n - 1) // 2
+        u, w = gx[o1:o1 + n], gw[o1:o1 + n]
+        ui = np.repeat(u, n)
+        wi = np.repeat(w, n)
+        uj = np.tile(u, n)
+        wj = np.tile(w, n)
+        out[0, off:off + n * n] = ui
+        out[1, off:off + n * n] = uj * (1.0 - ui)
+        out[2, off:off + n * n] = wi * wj * (1.0 - ui)
+        off += n * n
+    assert off == GQ_TOTAL
+    return out.reshape(-1)
+
+
 @functools.lru_cache(maxsize=1)
 def gauss_tables():
     """(nodes, weights) on [0, 1] for orders 1..192, order n at n(n-1)/2."""

@@ -124,6 +124,7 @@ class Device:
         gx, gw = geometry.gauss_tables()
         self.gl_x = torch.from_numpy(gx).to(self.dev)
         self.gl_w = torch.from_numpy(gw).to(self.dev)
+        self.gq = torch.from_numpy(geometry.gauss_triangle_tables()).to(self.dev)
         self._coef = {}
 
     def coef(self, mu: float, lam: float):
@@ -228,6 +229,7 @@ class Prepared:
             obs_lo=lo, obs_hi=hi, mu=job.mu, lam=job.lam,
             obs=_ptr(self.obs_t), nrm=_ptr(self.nrm_t), rb=_ptr(self.rb_t), sb=_ptr(self.sb_t),
             src=_ptr(self.src_t), coef=_ptr(self.coef_t), gl_x=_ptr(dev.gl_x), gl_w=_ptr(dev.gl_w),
+            gq=_ptr(dev.gq),
             out=out_p, out2=out2_p, strad=_ptr(self.strad_t), n_strad=self.n_strad,
             fold_list=_ptr(self.fold_list), fold_count=_ptr(self.fold_cnt), fold_err=_ptr(self.fold_err),
             fold_cap=self.fold_cap, stages=stages)

@@ -30,7 +30,13 @@ def test_sizes(lib):
     from mhs_amd import geometry
     assert lib.mhs_src_stride() == geometry.SRC_STRIDE
     assert lib.mhs_ncoef() == 2167
-    assert lib.mhs_abi_version() == 1
+    from mhs_amd import _lib
+    assert lib.mhs_abi_version() == _lib.ABI_VERSION == 2
+    # the launch descriptor mirror has the header's layout: 31 fields, the
+    # flattened Gauss rule pointer right after the 1-D tables
+    names = [f[0] for f in _lib.Launch._fields_]
+    assert names.index("gq") == names.index("gl_w") + 1
+    assert geometry.GQ_TOTAL == sum(n * n for n in range(1, 193))
 
 
 def test_bad_descriptors_refused_on_host(lib):
@@ -40,7 +46,7 @@ def test_bad_descriptors_refused_on_host(lib):
     L = _lib.Launch(abi_version=99)
     assert lib.mhs_assemble(C.byref(L), None) != 0
     assert b"ABI" in lib.mhs_last_error()
-    L = _lib.Launch(abi_version=1, order=5)
+    L = _lib.Launch(abi_version=_lib.ABI_VERSION, order=5)
     assert lib.mhs_assemble(C.byref(L), None) != 0
     assert b"order" in lib.mhs_last_error()
     assert lib.mhs_fill_material(0.0, 1.0, None) != 0

@@ -155,3 +155,18 @@ def test_precision_setting():
         mhs_amd.set_precision("fp16")
     with pytest.raises(ValueError):
         mhs_amd.precision("half")
+
+
+def test_flattened_gauss_rules_match_the_nested_rule():
+    """The flattened collapsed-Gauss table the far pass reads equals the
+    oracle's nested rule (moments.py:427-498) point for point."""
+    from mhs_amd import geometry
+    from oracle.geometry import gauss_triangle
+    t = geometry.gauss_triangle_tables().reshape(3, -1)
+    for n in (1, 2, 8, 12, 37, 192):
+        off = (n - 1) * n * (2 * n - 1) // 6
+        x1, x2, w = gauss_triangle(n)
+        assert np.array_equal(t[0, off:off + n * n], x1)
+        assert np.array_equal(t[1, off:off + n * n], x2)
+        assert np.allclose(t[2, off:off + n * n], w, rtol=1e-15, atol=0)
+        assert abs(t[2, off:off + n * n].sum() - 0.5) < 1e-14
