@@ -149,6 +149,7 @@ def _time_mode(mixed, steps, warmup):
             stage_ms[name] = e0.elapsed_time(e1)
         prep.launch()                    # leave a whole result for the error check
         torch.cuda.synchronize()
+    stage_ms["far_tiles"] = prep.far_tiles
     return prep, dt, stage_ms
 
 

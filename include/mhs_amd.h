@@ -76,7 +76,8 @@ typedef struct mhs_launch {
   int32_t* fold_err;        /* 1 int32: bit 0 grid overflow, bit 1 list overflow (caller zeroes) */
   int64_t fold_cap;
   int32_t stages;           /* 0 = all passes; else a mask of MHS_STAGE_* (timing, serving) */
-  int32_t reserved;
+  int32_t far_tiles;        /* column tiles per block of the contracted far pass (0 = 16);
+                               only the grouping of pairs into blocks, never the values */
 } mhs_launch;
 
 /* Assemble one matrix block: replaces the per-source loops of

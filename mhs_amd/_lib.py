@@ -31,7 +31,7 @@ class Launch(C.Structure):
         ("out", C.c_void_p), ("out2", C.c_void_p),
         ("strad", C.c_void_p), ("n_strad", C.c_int64),
         ("fold_list", C.c_void_p), ("fold_count", C.c_void_p), ("fold_err", C.c_void_p),
-        ("fold_cap", C.c_int64), ("stages", C.c_int32), ("reserved", C.c_int32),
+        ("fold_cap", C.c_int64), ("stages", C.c_int32), ("far_tiles", C.c_int32),
     ]
 
 

@@ -31,7 +31,7 @@
 #define MHS_FAR_WAVES64 2 // occupancy target of the far pass, FP64 points
 #endif
 #ifndef MHS_QC_CT
-#define MHS_QC_CT 16   // column tiles per block of the far pass
+#define MHS_QC_CT 16   // default column tiles per block of the far pass
 #endif
 
 static thread_local char g_err[512];
@@ -152,7 +152,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 //
 // BR, BQ (the image records contracted with E, BR with the receiver-depth
 // powers of its coefficients) are built once per block: a block is one
-// receiver row and MHS_QC_CT column tiles of PPB pairs.  LPP = 64 / NPT lanes
+// receiver row and pr.far_tiles column tiles of PPB pairs.  LPP = 64 / NPT lanes
 // per pair, NPT points per lane per 64-point chunk; the kappas live in LDS and
 // every read feeds NPT FMAs.  T is the point arithmetic (double, or float in
 // the mixed mode); point geometry, weights and all sums stay in double.  The
@@ -319,8 +319,8 @@ k_far(Params pr) {
   __syncthreads();
   const double C1 = 1.0 / (16.0 * M_PI * pr.mu * (1.0 - pr.nu));
   const double c34 = 3.0 - 4.0 * pr.nu;
-  for (int tile = 0; tile < MHS_QC_CT; ++tile) {
-    const int64_t col = ((int64_t)blockIdx.x * MHS_QC_CT + tile) * PPB + grp;
+  for (int tile = 0; tile < pr.far_tiles; ++tile) {
+    const int64_t col = ((int64_t)blockIdx.x * pr.far_tiles + tile) * PPB + grp;
     const bool valid = col < pr.n_cols;
     int64_t s = 0;
     int k = 0, nd = 0, ni = 0, nq = 0;
@@ -727,7 +727,7 @@ static int launch_all(const mhs_launch* L, const Params& pr, hipStream_t st) {
       // far pass (writes every entry), then the near pass (adds)
       if (stages & MHS_STAGE_Q) {
         auto fg = [&](int npt) {   // pairs per block = 256 / (64 / npt)
-          const int64_t per = (int64_t)(4 * npt) * MHS_QC_CT;
+          const int64_t per = (int64_t)(4 * npt) * pr.far_tiles;
           return dim3((unsigned)((pr.n_cols + per - 1) / per), (unsigned)rows);
         };
         if (nout == 1 && mixed) hipLaunchKernelGGL((k_far<float, P, 1, N32>), fg(N32), block, 0, st, pc);
@@ -815,6 +815,8 @@ int mhs_assemble(const mhs_launch* L, void* stream) {
   pr.out_lo = L->obs_lo;
   pr.kind = L->kind; pr.flags = L->flags; pr.n_a = L->n_a; pr.n_b = L->n_b;
   pr.mu = L->mu; pr.lam = L->lam; pr.nu = 0.5 * L->lam / (L->lam + L->mu);
+  if (L->far_tiles < 0 || L->far_tiles > 4096) return fail("mhs_assemble: far_tiles out of range");
+  pr.far_tiles = L->far_tiles ? L->far_tiles : MHS_QC_CT;
   hipStream_t st = (hipStream_t)stream;
   switch (L->order) {
     case 0: return launch_all<0>(L, pr, st);

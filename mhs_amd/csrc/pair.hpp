@@ -25,6 +25,7 @@ struct Params {
   int64_t obs_lo;        // first observer row of this launch
   int64_t out_lo;        // observer row held by the first row of out / out2
   int kind, flags, n_a, n_b;
+  int far_tiles;         // column tiles per block of the contracted far pass
   double mu, lam, nu;
 };
 

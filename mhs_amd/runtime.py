@@ -14,6 +14,9 @@ import numpy as np
 from . import _lib, geometry
 
 ROW_SCRATCH_BYTES = 4 << 30      # continuous-layout element-major scratch per launch
+FAR_TILE_CANDIDATES = (4, 8, 16, 32)   # column tiles per block tried by the autotuner
+AUTOTUNE_PAIRS = 1 << 22               # pairs of the sample each candidate is timed on
+autotune = True                         # off: the library default (16) is used
 
 PRECISIONS = ("fp64", "mixed")
 _precision = "fp64"
@@ -126,6 +129,7 @@ class Device:
         self.gl_w = torch.from_numpy(gw).to(self.dev)
         self.gq = torch.from_numpy(geometry.gauss_triangle_tables()).to(self.dev)
         self._coef = {}
+        self.far_tiles = {}     # autotuner cache: (order, outputs, mixed, log2 columns) -> tiles
 
     def coef(self, mu: float, lam: float):
         key = (float(mu), float(lam))
@@ -220,6 +224,11 @@ class Prepared:
         self.flags = ((_lib.F_FOLD if job.fold else 0) | (_lib.F_SUBTRACT if job.subtract else 0)
                       | (_lib.F_PARTS if job.parts else 0) | (_lib.F_MIXED if job.mixed else 0))
         self.n_b_launch = 3 if (cont and job.kind == "interaction") else job.n_b
+        # the contracted far pass (interaction, one or two components) has a
+        # tunable launch shape: column tiles per block
+        self.contracted = (job.kind == "interaction" and not cont and job.sb is not None
+                           and job.n_a * job.n_b <= 2)
+        self.far_tiles = 0
 
     def _launch_struct(self, lo, hi, out_p, out2_p, stages):
         job, dev = self.job, self.dev
@@ -232,7 +241,7 @@ class Prepared:
             gq=_ptr(dev.gq),
             out=out_p, out2=out2_p, strad=_ptr(self.strad_t), n_strad=self.n_strad,
             fold_list=_ptr(self.fold_list), fold_count=_ptr(self.fold_cnt), fold_err=_ptr(self.fold_err),
-            fold_cap=self.fold_cap, stages=stages)
+            fold_cap=self.fold_cap, stages=stages, far_tiles=self.far_tiles)
 
     def chunks(self, max_pairs: int | None = None):
         """Observer-row ranges that together cover the job, each at most
@@ -244,10 +253,50 @@ class Prepared:
             step = max(1, min(step, int(max_pairs) // max(self.n_cols, 1)))
         return [(lo, min(job.n_obs, lo + step)) for lo in range(0, job.n_obs, step)]
 
+    def tune_key(self):
+        return (self.job.order, self.job.n_a * self.job.n_b, bool(self.job.mixed),
+                int(np.log2(max(self.n_cols, 1))))
+
+    def tune(self):
+        """Pick the far pass's column tiles per block: each candidate is timed
+        (HIP events, after one warm launch) on the far pass over a sample of
+        the job's rows; the fastest is cached per (order, outputs, precision,
+        size class) on the device.  Only the grouping of pairs into blocks
+        changes, never a value; the sample rows are rewritten by the real
+        launch.  Synchronises the stream once per new key."""
+        if not self.contracted or self.far_tiles:
+            return self.far_tiles
+        key = self.tune_key()
+        dev, torch = self.dev, self.dev.torch
+        if key in dev.far_tiles:
+            self.far_tiles = dev.far_tiles[key]
+            return self.far_tiles
+        if not autotune:
+            return 0
+        rows = int(min(self.job.n_obs, max(1, AUTOTUNE_PAIRS // max(self.n_cols, 1))))
+        times = {}
+        with torch.cuda.stream(self.stream):
+            for ct in FAR_TILE_CANDIDATES:
+                self.far_tiles = ct
+                self.launch_rows(0, rows, _lib.STAGE_Q)
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(self.stream)
+                self.launch_rows(0, rows, _lib.STAGE_Q)
+                e1.record(self.stream)
+                e1.synchronize()
+                times[ct] = e0.elapsed_time(e1)
+        best = min(times, key=times.get)
+        dev.far_tiles[key] = best
+        self.far_tiles = best
+        return best
+
     def launch_rows(self, lo: int, hi: int, stages: int = 0):
         """Enqueue every pass for observer rows [lo, hi) (at most ``self.rows``
-        of them in the continuous layout); no host synchronisation."""
+        of them in the continuous layout); no host synchronisation (except
+        the autotuner's first visit of a new launch shape)."""
         job, lib = self.job, self.dev.lib
+        if self.contracted and not self.far_tiles:
+            self.tune()
         if self.cont:
             if hi - lo > self.rows:
                 raise ValueError(f"rows [{lo}, {hi}) exceed the element scratch ({self.rows} rows)")

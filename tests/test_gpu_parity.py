@@ -202,3 +202,30 @@ def test_c3_full_matrix_one_call(M):
     assert err < 1e-6
     for lo in (0, 5000, 5024, 9998):
         assert relmax(Km[lo:lo + 2], g[f"rows_{lo}"]) < 1e-6, lo
+
+
+def test_far_tiles_do_not_change_values(M):
+    """The autotuned launch shape (column tiles per block of the far pass)
+    only regroups pairs: every candidate gives bitwise the same matrix, and
+    the tuner caches one of the candidates."""
+    import torch
+    from mhs_amd import api, runtime, workloads
+    tris, eps = workloads.c3_interaction()
+    for prec in ("fp64", "mixed"):
+        print(f"case far tiles {prec}", flush=True)
+        with M.precision(prec):
+            job = api.plan(M.interaction_matrix, tris, M.Material(30.0, 30.0), eps, receiver="strike",
+                           source="strike", rows=(4990, 5030))
+        outs = []
+        for ct in runtime.FAR_TILE_CANDIDATES:
+            prep = runtime.Prepared(job)
+            prep.far_tiles = ct
+            with torch.cuda.stream(prep.stream):
+                prep.launch()
+            torch.cuda.synchronize()
+            outs.append(prep.final.clone())
+        assert all(torch.equal(outs[0], o) for o in outs[1:])
+        prep = runtime.Prepared(job)
+        ct = prep.tune()
+        assert ct in runtime.FAR_TILE_CANDIDATES
+        assert prep.dev.far_tiles[prep.tune_key()] == ct
