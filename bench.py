@@ -84,6 +84,17 @@ def cpu_baseline(n_cols: int, cores: int) -> dict:
                       f"{dt:.1f} s wall on {used} processes"}
 
 
+def _traffic(mode: str):
+    """HBM bytes per far-pass launch from the committed counter runs
+    (FETCH_SIZE, WRITE_SIZE, each scaled by a copy / fill of known size), or
+    None when they were taken in the other mode or are absent."""
+    path = os.path.join(ROOT, "profiles", "k_far_mixed_traffic.json")
+    if mode != "mixed" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)["hbm_bytes_per_launch"]
+
+
 def _progress(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -214,7 +225,7 @@ def main() -> int:
                    "n": n, "pairs": pairs, "mode": args.mode},
         "roofline": {"bound": "valu", "kernel": f"k_far (direct + image R + Q far field, contracted, {args.mode})",
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": None},
+                     "traffic": _traffic(args.mode)},
         "stage_ms": stage_ms,
         "other_mode": {"mode": "fp64" if mixed else "mixed", "ms_per_step": dt_o * 1e3,
                        "value": pairs / dt_o, "stage_ms": stage_o},
