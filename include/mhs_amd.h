@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define MHS_ABI_VERSION 2
+#define MHS_ABI_VERSION 3
 
 #define MHS_GQ_TOTAL 2377760   /* sum of n^2 for n = 1..192 */
 
@@ -73,11 +73,15 @@ typedef struct mhs_launch {
   int64_t n_strad;
   int64_t* fold_list;       /* scratch: 2 x fold_cap int64 (receiver, source) pairs */
   int32_t* fold_count;      /* scratch: 2 int32 */
-  int32_t* fold_err;        /* 1 int32: bit 0 grid overflow, bit 1 list overflow (caller zeroes) */
+  int32_t* fold_err;        /* 1 int32: bit 0 grid overflow, bit 1 list overflow, bit 2 work
+                               buffer overflow (caller zeroes) */
   int64_t fold_cap;
   int32_t stages;           /* 0 = all passes; else a mask of MHS_STAGE_* (timing, serving) */
   int32_t far_tiles;        /* column tiles per block of the contracted far pass (0 = 16);
                                only the grouping of pairs into blocks, never the values */
+  void* fold_work;          /* scratch: fold_work_items x mhs_fold_item_bytes(order) bytes, the
+                               folded pairs' polar grids and centre loads (prepared in parallel) */
+  int64_t fold_work_items;  /* capacity in folded pairs; more pairs than this set fold_err bit 2 */
 } mhs_launch;
 
 /* Assemble one matrix block: replaces the per-source loops of
@@ -111,6 +115,9 @@ int mhs_strain(double* buf, int64_t n_obs, int64_t n_cols, double mu, double lam
  * the scale 2 (lam + mu) / (pi mu (lam + 2 mu)) folded in
  * (src/mhs/kernels/image.py:95-104, 276-297, 453-456). */
 int mhs_fill_material(double mu, double lam, double* host_coef);
+
+/* Bytes of one folded pair's slot in fold_work at this order. */
+int64_t mhs_fold_item_bytes(int order);
 
 int mhs_ncoef(void);
 int mhs_src_stride(void);

@@ -10,7 +10,7 @@ import ctypes as C
 import pathlib
 
 LIB_PATH = pathlib.Path(__file__).resolve().parent / "libmhs_amd.so"
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 KIND = {"disp": 0, "stress": 1, "eigen": 2, "traction": 3, "interaction": 4}
 F_FOLD, F_SUBTRACT, F_PARTS, F_MIXED = 1, 2, 4, 8
@@ -32,13 +32,14 @@ class Launch(C.Structure):
         ("strad", C.c_void_p), ("n_strad", C.c_int64),
         ("fold_list", C.c_void_p), ("fold_count", C.c_void_p), ("fold_err", C.c_void_p),
         ("fold_cap", C.c_int64), ("stages", C.c_int32), ("far_tiles", C.c_int32),
+        ("fold_work", C.c_void_p), ("fold_work_items", C.c_int64),
     ]
 
 
 _LIB = None
 
 EXPORTS = ("mhs_assemble", "mhs_histogram", "mhs_gather", "mhs_strain", "mhs_fill_material", "mhs_ncoef",
-           "mhs_src_stride", "mhs_abi_version", "mhs_last_error")
+           "mhs_src_stride", "mhs_abi_version", "mhs_last_error", "mhs_fold_item_bytes")
 
 
 def load(path=None):
@@ -63,6 +64,8 @@ def load(path=None):
     lib.mhs_strain.restype = C.c_int
     lib.mhs_fill_material.argtypes = [C.c_double, C.c_double, C.c_void_p]
     lib.mhs_fill_material.restype = C.c_int
+    lib.mhs_fold_item_bytes.argtypes = [C.c_int]
+    lib.mhs_fold_item_bytes.restype = C.c_int64
     for nm in ("mhs_ncoef", "mhs_src_stride", "mhs_abi_version"):
         getattr(lib, nm).restype = C.c_int
     lib.mhs_last_error.restype = C.c_char_p

@@ -76,6 +76,14 @@ struct FoldGrid {
   int overflow;
 };
 
+// One folded pair's slot of the fold work buffer: its polar grid and the
+// surface load at the receiver's foot for every node, prepared in parallel
+// (a thread per pair) before the point sums.
+struct FoldItem {
+  FoldGrid G;
+  double s0[6][3];
+};
+
 // numpy linspace node i of (start, stop, num): i * step + start, stop exact.
 MHS_INLINE double lin_node(double start, double stop, int num, int i) {
   if (i == num - 1) return stop;
@@ -241,6 +249,40 @@ MHS_INLINE void surface_load(const Params& pr, const double* __restrict__ src, i
   const double z = weighted_tables<SpecH<P>, P>(f, src + S_V, k, o, eps, pr.gl_x, pr.gl_w, W, X1, X2);
   // sigma_zz (receiver half of e_z on the plane z = 0) per unit slip e_m,
   // minus the eigenstress E C_zz, by the contracted lift
+  const double ez[3] = {0.0, 0.0, 1.0};
+  double E[6];
+  half_tensor(ez, ez, pr.mu, pr.lam, E);
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const double em[3] = {m == 0 ? 1.0 : 0.0, m == 1 ? 1.0 : 0.0, m == 2 ? 1.0 : 0.0};
+    double Fm[6];
+    half_tensor(em, src + S_N, pr.mu, pr.lam, Fm);
+    s[m] = 2.0 * direct_contract(f, W, z, eps, mt, E, Fm, true, Fm[2]);
+  }
+}
+
+
+// The same surface load at a point known to be far from the triangle
+// (sqrt(|x - c|^2 + eps^2) > 10 L): the collapsed Gauss tables only, so a
+// kernel evaluating far points alone carries none of the closed form's
+// registers.
+template <int P>
+MHS_INLINE void surface_load_far(const Params& pr, const double* __restrict__ src, int k, double x, double y,
+                                 double s[3]) {
+  const double o[3] = {x, y, 0.0};
+  const double eps = src[S_EPS];
+  const Mat mt{pr.mu, pr.lam, pr.nu};
+  Frame f;
+  load_frame(src, false, f);
+  const double d0 = o[0] - f.c[0], d1 = o[1] - f.c[1], d2 = o[2] - f.c[2];
+  const double z = d0 * f.n[0] + d1 * f.n[1] + d2 * f.n[2];
+  const double X1 = d0 * f.e1[0] + d1 * f.e1[1] + d2 * f.e1[2];
+  const double X2 = d0 * f.e2[0] + d1 * f.e2[1] + d2 * f.e2[2];
+  const double h2 = z * z + eps * eps;
+  const int ng = centroid_distance(f.c, o, eps) > 40.0 * f.L ? 8 : 12;
+  const int off = ng * (ng - 1) / 2;
+  double W[6][21];
+  weighted_far<SpecH<P>, P>(f, k, X1, X2, h2, ng, pr.gl_x + off, pr.gl_w + off, W);
   const double ez[3] = {0.0, 0.0, 1.0};
   double E[6];
   half_tensor(ez, ez, pr.mu, pr.lam, E);

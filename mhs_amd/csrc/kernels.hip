@@ -1,16 +1,19 @@
 // gfx950 kernels of the mollified Mindlin half-space assembler and the
 // C-ABI they are launched through (declared in include/mhs_amd.h).
 //
-// Launch plan of one matrix call (reference: src/mhs/kernels/assemble.py):
-//   k_pair   one thread per (observer, source node): direct + image R-family
-//            (or the folded mirror in the band) + eigenstress, written in the
-//            output's own layout;
-//   k_q      one thread per (observer, source node): the image Q-family at the
-//            budget-law Gauss order, added;
-//   k_fold_* one workgroup per folded (receiver, straddling source) pair: the
-//            Boussinesq surface integral, added;
-//   k_gather the continuous layout's shared-column sums; k_strain the elastic
-//            strain readout.
+// Launch plan of one matrix call (reference: src/mhs/kernels/assemble.py),
+// observer rows in chunks, a 2-D grid (row, column tile) per kernel:
+//   interaction with one or two components (contracted):
+//     k_far   the far field (direct, image R, image Q) as per-pair contracted
+//             pointwise series, written;
+//     k_near  the closed-form near field and the folded mirror, added;
+//   every other form:
+//     k_pair  direct + image R-family (or the folded mirror) + eigenstress,
+//             written in the output's layout;
+//     k_qm    the image Q-family, moments first, added;
+//   then k_fold_list / k_fold_prep / k_fold (near and far points): the
+//   Boussinesq surface integral of the folded pairs, added; k_gather the
+//   continuous layout's shared-column sums; k_strain the elastic strain.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <string.h>
@@ -29,6 +32,9 @@
 #endif
 #ifndef MHS_FAR_WAVES64
 #define MHS_FAR_WAVES64 2 // occupancy target of the far pass, FP64 points
+#endif
+#ifndef MHS_FOLD_FAR_WAVES
+#define MHS_FOLD_FAR_WAVES 2   // occupancy target of the fold's far-point kernel
 #endif
 #ifndef MHS_QC_CT
 #define MHS_QC_CT 16   // default column tiles per block of the far pass
@@ -500,39 +506,75 @@ __global__ void k_fold_list(Params pr, const int64_t* __restrict__ strad, int64_
   }
 }
 
-template <int P, bool STRESS>
-__global__ void __launch_bounds__(256) k_fold(Params pr, const int64_t* __restrict__ list,
+// Prepare every folded pair in parallel, a thread each: its polar grid and
+// the surface load at the receiver's foot per node, into the work buffer.
+template <int P>
+__global__ void __launch_bounds__(64) k_fold_prep(Params pr, const int64_t* __restrict__ list,
+                                                  const int* __restrict__ count, int64_t cap,
+                                                  FoldItem* __restrict__ work, int64_t work_items,
+                                                  int* __restrict__ err) {
+  constexpr int K = (P + 1) * (P + 2) / 2;
+  int64_t n_items = *count;
+  if (n_items > cap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(err, 2);
+    n_items = cap;
+  }
+  if (n_items > work_items) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(err, 4);
+    n_items = work_items;
+  }
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; item < n_items; item += stride) {
+    const int64_t o = list[2 * item], s = list[2 * item + 1];
+    const double* src = pr.src + s * MHS_SRC_STRIDE;
+    const double xh0 = pr.obs[3 * o], xh1 = pr.obs[3 * o + 1];
+    const double zeta = fmax(0.0, -pr.obs[3 * o + 2]);
+    FoldItem& it = work[item];
+    fold_grid_build(it.G, xh0, xh1, zeta, src[S_CREASE], src[S_CREASE + 1], src[S_CREASE_R], src[S_EPS],
+                    src[S_DIAM]);
+    if (it.G.overflow) atomicOr(err, 1);
+    for (int k = 0; k < K; ++k) surface_load<P>(pr, src, k, xh0, xh1, it.s0[k]);
+  }
+}
+
+// The Boussinesq sum of the folded pairs over their polar grids, split by
+// point: NEAR the points within the closed-form range of the source
+// (sqrt(|x - c|^2 + eps^2) <= 10 L) plus Love's disk term, FAR the others by
+// the Gauss tables only (a kernel without the closed form's registers).
+// Each adds its part; together they are the whole sum.
+template <int P, bool STRESS, bool NEAR>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEAR ? 1 : MHS_FOLD_FAR_WAVES)))
+k_fold(Params pr, const int64_t* __restrict__ list,
                                               const int* __restrict__ count, int64_t cap,
-                                              int* __restrict__ err) {
+                                              const FoldItem* __restrict__ work, int64_t work_items) {
   constexpr int K = (P + 1) * (P + 2) / 2;
   constexpr int NA = STRESS ? 18 : 9;
+  constexpr int GW = sizeof(FoldGrid) / 8;
+  static_assert(sizeof(FoldGrid) % 8 == 0, "FoldGrid copies as 8-byte words");
   __shared__ FoldGrid G;
   __shared__ double sh[3];
   __shared__ double red[NA][8];
   __shared__ double u6[6], w6[6], u4[4], w4[4];
   const int tid = threadIdx.x;
-  int n_items = *count;
-  if (n_items > cap) {
-    if (blockIdx.x == 0 && tid == 0) atomicOr(err, 2);
-    n_items = (int)cap;
-  }
+  int64_t n_items = *count;
+  if (n_items > cap) n_items = cap;
+  if (n_items > work_items) n_items = work_items;
   if (tid < 6) { u6[tid] = pr.gl_x[15 + tid]; w6[tid] = pr.gl_w[15 + tid]; }
   if (tid < 4) { u4[tid] = pr.gl_x[6 + tid]; w4[tid] = pr.gl_w[6 + tid]; }
-  for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+  for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
     const int64_t o = list[2 * item], s = list[2 * item + 1];
     const double* src = pr.src + s * MHS_SRC_STRIDE;
-    const double eps = src[S_EPS];
     const double xh0 = pr.obs[3 * o], xh1 = pr.obs[3 * o + 1];
     const double zeta = fmax(0.0, -pr.obs[3 * o + 2]);
     __syncthreads();
-    if (tid == 0) {
-      fold_grid_build(G, xh0, xh1, zeta, src[S_CREASE], src[S_CREASE + 1], src[S_CREASE_R], eps,
-                      src[S_DIAM]);
-      if (G.overflow) atomicOr(err, 1);
+    {
+      const unsigned long long* gs = reinterpret_cast<const unsigned long long*>(&work[item].G);
+      unsigned long long* gd = reinterpret_cast<unsigned long long*>(&G);
+      for (int w = tid; w < GW; w += blockDim.x) gd[w] = gs[w];
     }
     __syncthreads();
     for (int k = 0; k < K; ++k) {
-      if (tid == 0) surface_load<P>(pr, src, k, xh0, xh1, sh);
+      if (tid < 3) sh[tid] = work[item].s0[k][tid];
       __syncthreads();
       double acc[NA];
 #pragma unroll
@@ -541,8 +583,11 @@ __global__ void __launch_bounds__(256) k_fold(Params pr, const int64_t* __restri
       for (int g = tid; g < G.total; g += blockDim.x) {
         double x, y, w;
         fold_grid_point(G, xh0, xh1, u6, w6, u4, w4, g, x, y, w);
+        const double op[3] = {x, y, 0.0};
+        if ((centroid_distance(src + S_C, op, src[S_EPS]) <= 10.0 * src[S_L]) != NEAR) continue;
         double sl[3];
-        surface_load<P>(pr, src, k, x, y, sl);
+        if (NEAR) surface_load<P>(pr, src, k, x, y, sl);
+        else surface_load_far<P>(pr, src, k, x, y, sl);
         sl[0] -= s0; sl[1] -= s1; sl[2] -= s2;
         const double dx = xh0 - x, dy = xh1 - y;
         if (STRESS) {
@@ -579,8 +624,8 @@ __global__ void __launch_bounds__(256) k_fold(Params pr, const int64_t* __restri
         }
         const int64_t col = s * K + k;
         if (STRESS) {
-          double srr, szz;
-          disk_stress(G.R1, zeta, pr.nu, srr, szz);
+          double srr = 0.0, szz = 0.0;
+          if (NEAR) disk_stress(G.R1, zeta, pr.nu, srr, szz);
           // components (xx, yy, xy, xz, yz, zz) -> H[a][b][m]
           double H[27];
           const int ci[3][3] = {{0, 2, 3}, {2, 1, 4}, {3, 4, 5}};
@@ -592,7 +637,7 @@ __global__ void __launch_bounds__(256) k_fold(Params pr, const int64_t* __restri
               }
           emit_stress(pr, o, col, s, H, 0.0, 0.0, src + S_N, src + S_N, true);
         } else {
-          const double uz = disk_uz(G.R1, zeta, pr.mu, pr.nu);
+          const double uz = NEAR ? disk_uz(G.R1, zeta, pr.mu, pr.nu) : 0.0;
           double U[9];
           for (int i = 0; i < 3; ++i)
             for (int m = 0; m < 3; ++m) U[i * 3 + m] = tot[i * 3 + m] + (i == 2 ? sh[m] * uz : 0.0);
@@ -763,9 +808,20 @@ static int launch_all(const mhs_launch* L, const Params& pr, hipStream_t st) {
     hipLaunchKernelGGL(k_fold_list, dim3(grid_1d(nt)), dim3(256), 0, st, pr, L->strad,
                        L->n_strad, n_rows, L->fold_list, L->fold_count, L->fold_cap);
     if (check(hipGetLastError(), "k_fold_list launch")) return -1;
+    if (!L->fold_work || L->fold_work_items < 1) return fail("mhs_assemble: fold work buffer missing");
+    FoldItem* work = (FoldItem*)L->fold_work;
+    const int64_t prep_blocks = (L->fold_work_items + 63) / 64;
+    hipLaunchKernelGGL((k_fold_prep<P>), dim3((unsigned)(prep_blocks < 4096 ? prep_blocks : 4096)), dim3(64), 0, st,
+                       pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items, L->fold_err);
+    if (check(hipGetLastError(), "k_fold_prep launch")) return -1;
     const dim3 fg(2048);
-    if (stress) hipLaunchKernelGGL((k_fold<P, true>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, L->fold_err);
-    else hipLaunchKernelGGL((k_fold<P, false>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, L->fold_err);
+    if (stress) {
+      hipLaunchKernelGGL((k_fold<P, true, true>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items);
+      hipLaunchKernelGGL((k_fold<P, true, false>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items);
+    } else {
+      hipLaunchKernelGGL((k_fold<P, false, true>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items);
+      hipLaunchKernelGGL((k_fold<P, false, false>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items);
+    }
     if (check(hipGetLastError(), "k_fold launch")) return -1;
   }
   return 0;
@@ -779,6 +835,11 @@ const char* mhs_last_error(void) { return g_err; }
 int mhs_abi_version(void) { return MHS_ABI_VERSION; }
 
 int mhs_ncoef(void) { return MHS_NCOEF; }
+
+int64_t mhs_fold_item_bytes(int order) {
+  (void)order;
+  return (int64_t)sizeof(FoldItem);
+}
 
 int mhs_src_stride(void) { return MHS_SRC_STRIDE; }
 

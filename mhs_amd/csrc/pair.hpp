@@ -204,36 +204,6 @@ MHS_INLINE int q_order_fast(const double* __restrict__ src, const double o[3]) {
   return q_order<P>(src, o);
 }
 
-// Q-family sum for node k (image.py:315-377): accumulates DDG (stress) or DG.
-template <int P, bool STRESS>
-MHS_INLINE void q_family(const Params& pr, const double* __restrict__ src, int k, const double o[3],
-                         int nq, double* acc) {
-  const double* v = src + S_V;
-  const double eps2 = src[S_EPS] * src[S_EPS];
-  const double ar2 = 2.0 * src[S_AREA];
-  const int off = nq * (nq - 1) / 2;
-  const double* gx = pr.gl_x + off;
-  const double* gw = pr.gl_w + off;
-  for (int i = 0; i < nq; ++i) {
-    const double gi = gx[i], wi = gw[i];
-    for (int j = 0; j < nq; ++j) {
-      const double x1 = gi, x2 = gx[j] * (1.0 - gi);
-      const double w = wi * gw[j] * (1.0 - gi) * ar2;
-      const double l0 = 1.0 - x1 - x2;
-      const double y0 = l0 * v[0] + x1 * v[3] + x2 * v[6];
-      const double y1 = l0 * v[1] + x1 * v[4] + x2 * v[7];
-      const double y2 = l0 * v[2] + x1 * v[5] + x2 * v[8];
-      const double wn = w * shape_at<P>(k, l0, x1, x2);
-      const double d1 = o[0] - y0, d2 = o[1] - y1, d3 = o[2] + y2;
-      const double r2 = sqrt(d1 * d1 + d2 * d2 + d3 * d3 + eps2);
-      const double ir = 1.0 / r2;
-      const double iq = 1.0 / (r2 - d3);
-      if (STRESS) q_ddg_point(d1, d2, d3, ir, iq, wn, pr.coef, acc);
-      else q_dg_point(d1, d2, d3, ir, iq, wn, pr.coef, acc);
-    }
-  }
-}
-
 // ------------------------------------------------------------- epilogues --
 // Write (or add) the total stress H of one (observer row o, column col) into
 // the output form of pr.kind.  eig terms: phi_d with the element normal nd,
@@ -443,32 +413,3 @@ MHS_INLINE void pair_near_item(const Params& pr, int64_t o, int64_t col) {
   for (int t = 0; t < n_a * n_b; ++t) pr.out[(oo * pr.n_cols + col) * (n_a * n_b) + t] += acc[t];
 }
 
-// One (observer row, element column) of the Q-family pass (added).
-template <int P, bool STRESS>
-MHS_INLINE void q_item(const Params& pr, int64_t o, int64_t col) {
-  constexpr int K = (P + 1) * (P + 2) / 2;
-  const int64_t s = col / K;
-  const int k = (int)(col - s * K);
-  const double* src = pr.src + s * MHS_SRC_STRIDE;
-  const double ob[3] = {pr.obs[3 * o], pr.obs[3 * o + 1], pr.obs[3 * o + 2]};
-  if (in_fold_band(src, ob, pr.flags)) return;       // the fold replaces R + Q there
-  const int nq = q_order<P>(src, ob);
-  const double* nrm = src + S_N;
-  if (STRESS) {
-    double DDG[81];
-#pragma unroll
-    for (int t = 0; t < 81; ++t) DDG[t] = 0.0;
-    q_family<P, true>(pr, src, k, ob, nq, DDG);
-    double H[27];
-    contract_H(DDG, nrm, pr.mu, pr.lam, H);
-    emit_stress(pr, o, col, s, H, 0.0, 0.0, nrm, nrm, true);
-  } else {
-    double DG[27];
-#pragma unroll
-    for (int t = 0; t < 27; ++t) DG[t] = 0.0;
-    q_family<P, false>(pr, src, k, ob, nq, DG);
-    double U[9];
-    contract_U(DG, nrm, pr.mu, pr.lam, U);
-    emit_disp(pr, o, col, U, true);
-  }
-}

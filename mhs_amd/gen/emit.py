@@ -11,7 +11,8 @@ and writes ``mhs_amd/csrc/gen_image.inc`` containing
   image-frame weighted moment tables W[n][u][v] to the Cartesian monomial
   moments int N D1^a D2^b D3^c / R2^n (D = -xi1 e1' - xi2 e2' + z' n') and
   accumulates the records into DDG (81) / DG (27);
-* ``q_ddg_point`` / ``q_dg_point``: the Q-family at one quadrature point.
+* ``q_blocked`` / ``r_blocked``: the contracted image series, register-blocked
+  over quadrature points; ``qmom_*``: the Q-family's moments-first form.
 
 The traversal is depth-first over the monomial tree, so only one root-to-leaf
 path of expansion polynomials is live at a time.
@@ -158,38 +159,13 @@ def emit_r(em: Emitter, name, records, n_ix, n_slots):
 
 
 def emit_q(em: Emitter, name, records, n_ix):
-    by_mono = defaultdict(list)
+    """Register the Q-family records' coefficients (one per record, in record
+    order); the kernels evaluate the family through the contracted and the
+    moments-first forms emitted below."""
     for rec in records:
-        slot = slot_of(rec[:n_ix])
-        a, b, c, n, q = rec[n_ix:n_ix + 5]
         terms = [(pnu, Fraction(num, den)) for pz, pnu, num, den in rec[n_ix + 5]]
         assert all(pz == 0 for pz, *_ in rec[n_ix + 5])
-        by_mono[(a, b, c, n, q)].append((slot, em.coef(terms)))
-    emax = [max(m[i] for m in by_mono) for i in range(5)]
-    em.w(f"__host__ __device__ __forceinline__ void {name}(double d1, double d2, double d3,")
-    em.w("    double ir, double iq, double w, const double* __restrict__ C, double* acc)")
-    em.w("{")
-    for nm, var, e in (("x", "d1", emax[0]), ("y", "d2", emax[1]), ("zz", "d3", emax[2]),
-                       ("r", "ir", emax[3]), ("q", "iq", emax[4])):
-        em.w(f"  const double {nm}1 = {var};")
-        for k in range(2, e + 1):
-            em.w(f"  const double {nm}{k} = {nm}{k - 1} * {var};")
-    dparts = sorted({m[:3] for m in by_mono})
-    rparts = sorted({m[3:] for m in by_mono})
-    for a, b, c in dparts:
-        f = ["w"] + [f"x{a}"] * (a > 0) + [f"y{b}"] * (b > 0) + [f"zz{c}"] * (c > 0)
-        em.w(f"  const double D{a}{b}{c} = {' * '.join(f)};")
-    for n, q in rparts:
-        f = [f"r{n}"] * (n > 0) + [f"q{q}"] * (q > 0)
-        em.w(f"  const double R{n}{q} = {' * '.join(f) if f else '1.0'};")
-    for m in sorted(by_mono):
-        a, b, c, n, q = m
-        em.w(f"  {{ const double mm = D{a}{b}{c} * R{n}{q};")
-        for slot, k in by_mono[m]:
-            em.w(f"    acc[{slot}] += C[{k}] * mm;")
-        em.w("  }")
-    em.w("}")
-    em.w()
+        em.coef(terms)
 
 
 SYM = {(0, 0): 0, (1, 1): 1, (2, 2): 2, (0, 1): 3, (1, 0): 3, (0, 2): 4, (2, 0): 4, (1, 2): 5, (2, 1): 5}

@@ -14,6 +14,7 @@ import numpy as np
 from . import _lib, geometry
 
 ROW_SCRATCH_BYTES = 4 << 30      # continuous-layout element-major scratch per launch
+FOLD_ITEM_BYTES = 20200          # upper bound of mhs_fold_item_bytes (checked by the ABI tests)
 FAR_TILE_CANDIDATES = (4, 8, 16, 32)   # column tiles per block tried by the autotuner
 AUTOTUNE_PAIRS = 1 << 22               # pairs of the sample each candidate is timed on
 autotune = True                         # off: the library default (16) is used
@@ -218,6 +219,11 @@ class Prepared:
             self.fold_cap = max(1, self.rows * self.n_strad)
             self.fold_list = (torch.empty(2 * self.fold_cap, dtype=torch.int64, device=dev.dev)
                               if strad.size else None)
+            # the folded pairs' grids and centre loads: sized by the exact host
+            # count of band pairs (any one launch holds at most all of them)
+            self.fold_items = fold_pairs_bound(job.obs, recs, strad) if strad.size else 0
+            self.fold_work = (torch.empty(self.fold_items * dev.lib.mhs_fold_item_bytes(job.order),
+                                          dtype=torch.uint8, device=dev.dev) if self.fold_items else None)
             self.fold_cnt = torch.zeros(2, dtype=torch.int32, device=dev.dev)
             self.fold_err = torch.zeros(1, dtype=torch.int32, device=dev.dev)
         self.kind = _lib.KIND[job.kind]
@@ -241,7 +247,8 @@ class Prepared:
             gq=_ptr(dev.gq),
             out=out_p, out2=out2_p, strad=_ptr(self.strad_t), n_strad=self.n_strad,
             fold_list=_ptr(self.fold_list), fold_count=_ptr(self.fold_cnt), fold_err=_ptr(self.fold_err),
-            fold_cap=self.fold_cap, stages=stages, far_tiles=self.far_tiles)
+            fold_cap=self.fold_cap, stages=stages, far_tiles=self.far_tiles,
+            fold_work=_ptr(self.fold_work), fold_work_items=self.fold_items)
 
     def chunks(self, max_pairs: int | None = None):
         """Observer-row ranges that together cover the job, each at most
@@ -355,7 +362,26 @@ class Prepared:
         if self.n_strad:
             err = int(self.fold_err.cpu()[0])
             if err:
-                raise RuntimeError(f"mhs_amd: fold grid/list overflow (flags {err})")
+                raise RuntimeError(f"mhs_amd: fold grid/list/work overflow (flags {err})")
+
+
+def fold_pairs_bound(obs: np.ndarray, recs: np.ndarray, strad: np.ndarray, chunk: int = 4096) -> int:
+    """An upper bound on the (observer, straddling source) pairs in the fold
+    band (fold.py:84-118: receiver shallower than 10 eps, within 60 eps of the
+    crease centre): the host count plus a margin for the pairs a last-ulp
+    difference could move across the band's edge."""
+    if strad.size == 0:
+        return 0
+    eps = recs[strad, geometry.S_EPS]
+    cx = recs[strad, geometry.S_CREASE]
+    cy = recs[strad, geometry.S_CREASE + 1]
+    n = 0
+    for lo in range(0, obs.shape[0], chunk):
+        o = obs[lo:lo + chunk]
+        depth_ok = (-o[:, 2:3]) < 10.0 * eps[None, :] * (1.0 + 1e-12)
+        reach = np.hypot(o[:, 0:1] - cx[None, :], o[:, 1:2] - cy[None, :])
+        n += int(np.count_nonzero(depth_ok & (reach < 60.0 * eps[None, :] * (1.0 + 1e-12))))
+    return n + 64
 
 
 def job_bytes(job: Job) -> int:
@@ -374,6 +400,10 @@ def job_bytes(job: Job) -> int:
         scratch = rows * row_bytes
     n_strad = int(np.sum(np.asarray(job.tris)[:, :, 2].max(axis=1) > -3.0 * np.asarray(job.eps)))
     fold = 16 * max(1, rows * n_strad) if job.fold and n_strad else 0
+    if fold:
+        recs = geometry.source_records(job.tris, job.eps)
+        strad = np.flatnonzero(recs[:, geometry.S_STRADDLE] != 0.0)
+        fold += FOLD_ITEM_BYTES * fold_pairs_bound(job.obs, recs, strad)
     inputs = 8 * (job.n_obs * 3 * (2 + job.n_a) + job.n_src * (geometry.SRC_STRIDE + 3 * max(job.n_b, 0)))
     return int(out + scratch + fold + inputs)
 
