@@ -95,6 +95,27 @@ def _traffic(mode: str):
         return json.load(f)["hbm_bytes_per_launch"]
 
 
+def counter_pass(mode: str) -> None:
+    """One far pass of the workload (autotuner off, default column tiles),
+    then a 256 MiB fill and a 256 MiB device copy: the program that hardware
+    counter runs (FETCH_SIZE, WRITE_SIZE) profile, the fill and copy being
+    the known-size calibration of those counters."""
+    import torch
+    from mhs_amd import _lib, runtime
+    runtime.autotune = False
+    prep = runtime.Prepared(_job(mode == "mixed"))
+    with torch.cuda.stream(prep.stream):
+        prep.launch(stages=_lib.STAGE_Q)
+    torch.cuda.synchronize()
+    a = torch.empty(1 << 25, dtype=torch.float64, device="cuda")
+    a.fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    _progress(f"counter pass done ({mode}, default column tiles per block)")
+
+
 def _progress(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -174,12 +195,18 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=512,
                     help="requests of the served stream (0 = skip it)")
+    ap.add_argument("--counter-pass", action="store_true",
+                    help="run one far pass of --mode plus a fill and a copy of 256 MiB, print nothing "
+                         "(the program profiled by the FETCH_SIZE / WRITE_SIZE counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the assembly shards by "
               f"receiver rows with no exchange, and no multi-GPU figure is produced here.",
               file=sys.stderr)
         return 2
+    if args.counter_pass:
+        counter_pass(args.mode)
+        return 0
 
     from mhs_amd import flops
     mixed = args.mode == "mixed"

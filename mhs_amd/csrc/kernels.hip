@@ -291,6 +291,7 @@ k_far(Params pr) {
   __shared__ T kq_s[PPB][NOUT * MHS_QM];
   __shared__ T kr_s[PPB][NOUT * MHS_RM];
   __shared__ T kd_s[PPB][NOUT * MHS_DC];
+  __shared__ double res_s[PPB * NOUT];      // a tile's results, stored as one contiguous run
   const int tid = threadIdx.x;
   const int grp = tid / LPP, li = tid % LPP;
   const int n_a = pr.n_a, n_b = pr.n_b;
@@ -460,12 +461,18 @@ k_far(Params pr) {
       res[t] = fma(-C1, acc_d[t], acc_i[t]);
       for (int off = LPP / 2; off > 0; off >>= 1) res[t] += __shfl_xor(res[t], off, LPP);
     }
-    if (valid && li == 0) {
-      const int64_t oo = o - pr.out_lo;
+    if (li == 0) {
 #pragma unroll
-      for (int t = 0; t < NOUT; ++t) pr.out[(oo * pr.n_cols + col) * NOUT + t] = res[t];
+      for (int t = 0; t < NOUT; ++t) res_s[grp * NOUT + t] = res[t];
     }
-    __syncthreads();     // the kappas are rewritten by the next tile
+    __syncthreads();     // the kappas are rewritten by the next tile; res_s is complete
+    {
+      // the tile's PPB consecutive columns are one contiguous run of the row
+      const int64_t col0 = ((int64_t)blockIdx.x * pr.far_tiles + tile) * PPB;
+      const int64_t oo = o - pr.out_lo;
+      const int64_t n_here = (pr.n_cols - col0 < PPB ? pr.n_cols - col0 : PPB) * NOUT;
+      if (tid < n_here) pr.out[(oo * pr.n_cols + col0) * NOUT + tid] = res_s[tid];
+    }
   }
 }
 

@@ -327,6 +327,23 @@ class Prepared:
         for lo, hi in self.chunks():
             self.launch_rows(lo, hi, stages)
 
+    def graph(self, stages: int = 0):
+        """Capture the whole launch sequence as one HIP graph on this job's
+        stream (which must not be the default stream); ``replay()`` re-runs
+        the assembly into the same buffers.  The sequence has no memset node
+        (counters are cleared by a fill kernel) and no host synchronisation;
+        the far pass's launch shape is tuned before the capture."""
+        torch = self.dev.torch
+        if self.stream == torch.cuda.default_stream(self.dev.dev):
+            raise ValueError("graph capture needs a Prepared on a non-default stream")
+        if self.contracted and not self.far_tiles:
+            self.tune()
+            torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=self.stream):
+            self.launch(stages)
+        return g
+
     def _gather(self, lo, hi):
         job, lib = self.job, self.dev.lib
         for s_t, f_t in ((self.scr, self.final), (self.scr2, self.final2)):

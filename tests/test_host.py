@@ -170,3 +170,16 @@ def test_flattened_gauss_rules_match_the_nested_rule():
         assert np.array_equal(t[1, off:off + n * n], x2)
         assert np.allclose(t[2, off:off + n * n], w, rtol=1e-15, atol=0)
         assert abs(t[2, off:off + n * n].sum() - 0.5) < 1e-14
+
+
+def test_fold_work_bound_covers_the_band():
+    """The host bound that sizes the fold work buffer is the band count of
+    the branch histogram (fold.py:84-118) plus its margin."""
+    from mhs_amd import flops, geometry, runtime, workloads
+    tris, eps = workloads.c3_interaction(nx=40, nz=6)
+    obs = M.collocation_points(tris, 0)
+    recs = geometry.source_records(tris, np.full(tris.shape[0], eps))
+    strad = np.flatnonzero(recs[:, geometry.S_STRADDLE] != 0.0)
+    h = flops.histogram(obs, tris, eps)
+    assert h["fold_pairs"] > 0
+    assert runtime.fold_pairs_bound(obs, recs, strad) == h["fold_pairs"] + 64
