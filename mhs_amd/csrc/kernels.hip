@@ -293,7 +293,6 @@ k_far(Params pr) {
   __shared__ T kd_s[PPB][NOUT * MHS_DC];
   __shared__ double res_s[PPB * NOUT];      // a tile's results, stored as one contiguous run
   const int tid = threadIdx.x;
-  const int grp = tid / LPP, li = tid % LPP;
   const int n_a = pr.n_a, n_b = pr.n_b;
   const int64_t o = pr.obs_lo + blockIdx.y;
   const double ob[3] = {pr.obs[3 * o], pr.obs[3 * o + 1], pr.obs[3 * o + 2]};
@@ -324,9 +323,14 @@ k_far(Params pr) {
     }
   }
   __syncthreads();
-  const double C1 = 1.0 / (16.0 * M_PI * pr.mu * (1.0 - pr.nu));
-  const double c34 = 3.0 - 4.0 * pr.nu;
   for (int tile = 0; tile < pr.far_tiles; ++tile) {
+    // the lane's pair and LDS addresses are derived afresh each tile (the
+    // empty asm redefines the lane index), so they are not held, or spilled,
+    // across the tile loop
+    unsigned lane = tid;
+    __asm__ volatile("" : "+v"(lane));
+    lane &= 255u;
+    const int grp = (int)(lane / LPP), li = (int)(lane % LPP);
     const int64_t col = ((int64_t)blockIdx.x * pr.far_tiles + tile) * PPB + grp;
     const bool valid = col < pr.n_cols;
     int64_t s = 0;
@@ -420,13 +424,13 @@ k_far(Params pr) {
           c[9] = (T)Fv[3]; c[10] = (T)Fv[4]; c[11] = (T)Fv[5];
           c[12] = (T)M[0]; c[13] = (T)M[4]; c[14] = (T)M[8];
           c[15] = (T)(0.5 * (M[1] + M[3])); c[16] = (T)(0.5 * (M[2] + M[6])); c[17] = (T)(0.5 * (M[5] + M[7]));
-          c[18] = (T)((1.0 - c34) * EF + trE * trF);
+          c[18] = (T)(pr.k3 * EF + trE * trF);
           c[19] = (T)(-3.0 * trF);
           c[20] = (T)(-3.0 * trE);
-          c[21] = (T)(-6.0 * (1.0 - pr.nu) * e2 * EF);
-          c[22] = (T)(30.0 * (1.0 - pr.nu) * e2);
-          c[23] = (T)((pr.flags & F_SUBTRACT) ? 15.0 * e2 * e2 / (8.0 * M_PI * C1) * rFn : 0.0);
-          c[24] = (T)(3.0 * c34 - 9.0);
+          c[21] = (T)(pr.k5e * e2 * EF);
+          c[22] = (T)(pr.k7c * e2);
+          c[23] = (T)((pr.flags & F_SUBTRACT) ? pr.k7e * (e2 * e2) * rFn : 0.0);
+          c[24] = (T)pr.k5c;
         }
       }
     }
@@ -458,7 +462,7 @@ k_far(Params pr) {
     double res[NOUT];
 #pragma unroll
     for (int t = 0; t < NOUT; ++t) {
-      res[t] = fma(-C1, acc_d[t], acc_i[t]);
+      res[t] = fma(-pr.C1, acc_d[t], acc_i[t]);
       for (int off = LPP / 2; off > 0; off >>= 1) res[t] += __shfl_xor(res[t], off, LPP);
     }
     if (li == 0) {
@@ -883,6 +887,15 @@ int mhs_assemble(const mhs_launch* L, void* stream) {
   pr.out_lo = L->obs_lo;
   pr.kind = L->kind; pr.flags = L->flags; pr.n_a = L->n_a; pr.n_b = L->n_b;
   pr.mu = L->mu; pr.lam = L->lam; pr.nu = 0.5 * L->lam / (L->lam + L->mu);
+  {
+    const double c34 = 3.0 - 4.0 * pr.nu;
+    pr.C1 = 1.0 / (16.0 * M_PI * pr.mu * (1.0 - pr.nu));
+    pr.k5e = -6.0 * (1.0 - pr.nu);
+    pr.k7c = 30.0 * (1.0 - pr.nu);
+    pr.k7e = 15.0 / (8.0 * M_PI * pr.C1);
+    pr.k3 = 1.0 - c34;
+    pr.k5c = 3.0 * c34 - 9.0;
+  }
   if (L->far_tiles < 0 || L->far_tiles > 4096) return fail("mhs_assemble: far_tiles out of range");
   pr.far_tiles = L->far_tiles ? L->far_tiles : MHS_QC_CT;
   hipStream_t st = (hipStream_t)stream;

@@ -27,6 +27,11 @@ struct Params {
   int kind, flags, n_a, n_b;
   int far_tiles;         // column tiles per block of the contracted far pass
   double mu, lam, nu;
+  // constants of the contracted direct series, computed once on the host so
+  // that the far pass holds them in scalar registers: C1 = 1/(16 pi mu (1-nu)),
+  // A5 coefficient -6(1-nu), A7 coefficient 30(1-nu), eigenstress scale
+  // 15/(8 pi C1), and 1 - c34, 3 c34 - 9 with c34 = 3 - 4 nu
+  double C1, k5e, k7c, k7e, k3, k5c;
 };
 
 // Fold classification of one (receiver, source) pair (fold.py:84-118): the
