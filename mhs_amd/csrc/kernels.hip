@@ -481,13 +481,65 @@ k_far(Params pr) {
 }
 
 // Near parts of the contracted interaction (closed forms, folded mirror),
-// added after k_far; a thread per pair, most of which have nothing near.
+// added after k_far.
+#define MHS_NEAR_TILE 2048
+
+// One pair's near parts as a real call: inlined into the list loop its
+// closed forms spill far more than as a function of its own.
+template <int P>
+__device__ __attribute__((noinline)) void near_item_call(const Params& pr, int64_t o, int64_t col) {
+  pair_near_item<P>(pr, o, col);
+}
+
 template <int P>
 __global__ void __launch_bounds__(256) k_near(Params pr) {
+  // a block is one receiver row x MHS_NEAR_TILE columns; about one pair in
+  // nine has a near part, so the block first lists its pairs that do (those
+  // with an image or mirror part first, then the direct-only ones) and then
+  // evaluates the list with every lane busy; each pair is one thread's
+  constexpr int K = (P + 1) * (P + 2) / 2;
+  __shared__ int list_s[2 * MHS_NEAR_TILE];   // image-part pairs at 0.., direct-only at MHS_NEAR_TILE..
+  __shared__ int cnt_s[2][4];
+  const int tid = threadIdx.x, wave = tid >> 6;
   const int64_t o = pr.obs_lo + blockIdx.y;
-  const int64_t col = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (col >= pr.n_cols) return;
-  pair_near_item<P>(pr, o, col);
+  const int64_t c0 = (int64_t)blockIdx.x * MHS_NEAR_TILE;
+  const double ob[3] = {pr.obs[3 * o], pr.obs[3 * o + 1], pr.obs[3 * o + 2]};
+  int n_img = 0, n_dir = 0;
+  for (int step = 0; step < MHS_NEAR_TILE; step += 256) {
+    const int64_t col = c0 + step + tid;
+    const int parts = col < pr.n_cols ? near_parts(pr, pr.src + (col / K) * MHS_SRC_STRIDE, ob) : 0;
+    const bool img = (parts & 2) != 0, dir = parts == 1;
+    const unsigned long long mi = __ballot(img), md = __ballot(dir);
+    if ((tid & 63) == 0) {
+      cnt_s[0][wave] = __popcll(mi);
+      cnt_s[1][wave] = __popcll(md);
+    }
+    __syncthreads();
+    int oi = n_img, od = n_dir, si = 0, sd = 0;
+#pragma unroll
+    for (int wv = 0; wv < 4; ++wv) {
+      oi += wv < wave ? cnt_s[0][wv] : 0;
+      od += wv < wave ? cnt_s[1][wv] : 0;
+      si += cnt_s[0][wv];
+      sd += cnt_s[1][wv];
+    }
+    if (img)
+      list_s[oi + __builtin_amdgcn_mbcnt_hi((unsigned)(mi >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mi, 0u))] =
+          step + tid;
+    if (dir)
+      list_s[MHS_NEAR_TILE + od +
+             __builtin_amdgcn_mbcnt_hi((unsigned)(md >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)md, 0u))] =
+          step + tid;
+    n_img += si;
+    n_dir += sd;
+    __syncthreads();          // cnt_s is rewritten by the next step
+    if (c0 + step + 256 >= pr.n_cols) break;
+  }
+  for (int i = tid; i < n_img + n_dir; i += 256) {
+    __asm__ volatile("" ::: "memory");   // no loads hoisted out of the pair loop
+    const int c = i < n_img ? list_s[i] : list_s[MHS_NEAR_TILE + i - n_img];
+    near_item_call<P>(pr, o, c0 + c);
+  }
 }
 
 // ------------------------------------------------------------------ fold --
@@ -833,7 +885,8 @@ static int launch_all(const mhs_launch* L, const Params& pr, hipStream_t st) {
         if (check(hipGetLastError(), "k_far launch")) return -1;
       }
       if (stages & MHS_STAGE_PAIR) {
-        hipLaunchKernelGGL((k_near<P>), grid, block, 0, st, pc);
+        const dim3 ng((unsigned)((pr.n_cols + MHS_NEAR_TILE - 1) / MHS_NEAR_TILE), (unsigned)rows);
+        hipLaunchKernelGGL((k_near<P>), ng, block, 0, st, pc);
         if (check(hipGetLastError(), "k_near launch")) return -1;
       }
       continue;

@@ -326,6 +326,17 @@ MHS_INLINE void pair_item(const Params& pr, int64_t o, int64_t col) {
   }
 }
 
+// Which near parts a pair of the contracted interaction has, by the same
+// tests as pair_near_item: bit 0 the closed-form direct term, bit 1 the
+// folded mirror (band) or the closed-form image R-family.
+MHS_INLINE int near_parts(const Params& pr, const double* __restrict__ src, const double ob[3]) {
+  const double eps = src[S_EPS];
+  const bool band = in_fold_band(src, ob, pr.flags);
+  const bool dnear = centroid_distance(src + S_C, ob, eps) <= 10.0 * src[S_L];
+  const bool rnear = !band && centroid_distance(src + S_IC, ob, eps) <= 10.0 * src[S_IL];
+  return (dnear ? 1 : 0) | ((band || rnear) ? 2 : 0);
+}
+
 // Near part of one (observer row o, element column col) of a contracted
 // interaction, added after the far pass: the closed-form direct term and its
 // eigenstress when the pair is near (sqrt(|x - c|^2 + eps^2) <= 10 L), and on
