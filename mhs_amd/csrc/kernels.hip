@@ -27,6 +27,23 @@
 #ifndef MHS_FAR_NPT64
 #define MHS_FAR_NPT64 2   // quadrature points per lane of the far pass, FP64 points
 #endif
+// Lanes per pair of the far pass: fewer lanes put more pairs in a wave, so
+// the per-pair set-up is issued once for more pairs, until the per-block
+// kappa tables in LDS cut the occupancy (measured at n = 10^4: 8 lanes beat
+// 16 by 13 % with FP32 points and by 20 % with FP64 points; 4 lanes lose
+// with FP32 points, LDS-limited to 2 waves per SIMD).
+#ifndef MHS_FAR_LPP32
+#define MHS_FAR_LPP32 8   // FP32 points, one output
+#endif
+#ifndef MHS_FAR_LPP64
+#define MHS_FAR_LPP64 8   // FP64 points, one output
+#endif
+#ifndef MHS_FAR_LPP32B
+#define MHS_FAR_LPP32B 16 // FP32 points, two outputs
+#endif
+#ifndef MHS_FAR_LPP64B
+#define MHS_FAR_LPP64B 32 // FP64 points, two outputs
+#endif
 #ifndef MHS_FAR_WAVES32
 #define MHS_FAR_WAVES32 3 // occupancy target of the far pass, FP32 points (waves per SIMD)
 #endif
@@ -158,9 +175,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 //
 // BR, BQ (the image records contracted with E, BR with the receiver-depth
 // powers of its coefficients) are built once per block: a block is one
-// receiver row and pr.far_tiles column tiles of PPB pairs.  LPP = 64 / NPT lanes
-// per pair, NPT points per lane per 64-point chunk; the kappas live in LDS and
-// every read feeds NPT FMAs.  T is the point arithmetic (double, or float in
+// receiver row and pr.far_tiles column tiles of PPB = 256 / LPP pairs.  LPP
+// lanes per pair, NPT points per lane per chunk of LPP * NPT points; the
+// per-pair set-up (classification, kappas, direct coefficients) runs once per
+// 64 / LPP pairs of a wave.  The kappas live in LDS and every read feeds NPT
+// FMAs.  T is the point arithmetic (double, or float in
 // the mixed mode); point geometry, weights and all sums stay in double.  The
 // near parts (closed forms, folded mirror) are added by k_near.
 #define MHS_DC 25   // per-output direct coefficients: E(6) F(6) sym(EF)(6) A3 A5a A5b A5e A7c A7e A5c
@@ -277,12 +296,12 @@ __device__ __forceinline__ void far_pass(const Params& pr, const double* __restr
   }
 }
 
-template <typename T, int P, int NOUT, int NPT>
+template <typename T, int P, int NOUT, int NPT, int LPP>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? MHS_FAR_WAVES32 : MHS_FAR_WAVES64)))
 k_far(Params pr) {
+  static_assert(LPP >= 4 && LPP <= 64 && (LPP & (LPP - 1)) == 0, "lanes per pair: a power of two in [4, 64]");
   constexpr int K = (P + 1) * (P + 2) / 2;
-  constexpr int LPP = 64 / NPT;
   constexpr int PPB = 256 / LPP;
   constexpr int QB = MHS_QM * 6, RB = MHS_RM * 6;
   __shared__ double BQ_s[NOUT][QB];
@@ -441,8 +460,9 @@ k_far(Params pr) {
     const T* kd = kd_s[grp];
     const T* kr = kr_s[grp];
     const T* kq = kq_s[grp];
-    // 12 x 12 rules in chunks of 3 points per lane (16 lanes: 144 = 3 x 48)
-    constexpr int N12 = LPP == 16 ? 3 : NPT;
+    // 12 x 12 rules in whole chunks: NPT points per lane where LPP * NPT
+    // divides 144, else chunks of 48 (16 lanes x 3 or 8 lanes x 6)
+    constexpr int N12 = 144 % (LPP * NPT) == 0 ? NPT : LPP == 16 ? 3 : LPP == 8 ? 6 : NPT;
     if (valid && dfar) {
       if (nd == 12)
         far_pass<T, P, NOUT, N12, LPP, false, true, false, false>(pr, src, k, ob, 12, li, kd, kr, kq, acc_d, acc_i);
@@ -874,14 +894,15 @@ static int launch_all(const mhs_launch* L, const Params& pr, hipStream_t st) {
     if (contracted) {
       // far pass (writes every entry), then the near pass (adds)
       if (stages & MHS_STAGE_Q) {
-        auto fg = [&](int npt) {   // pairs per block = 256 / (64 / npt)
-          const int64_t per = (int64_t)(4 * npt) * pr.far_tiles;
+        auto fg = [&](int lpp) {   // pairs per block = 256 / lanes per pair
+          const int64_t per = (int64_t)(256 / lpp) * pr.far_tiles;
           return dim3((unsigned)((pr.n_cols + per - 1) / per), (unsigned)rows);
         };
-        if (nout == 1 && mixed) hipLaunchKernelGGL((k_far<float, P, 1, N32>), fg(N32), block, 0, st, pc);
-        else if (nout == 1) hipLaunchKernelGGL((k_far<double, P, 1, N64>), fg(N64), block, 0, st, pc);
-        else if (mixed) hipLaunchKernelGGL((k_far<float, P, 2, N32b>), fg(N32b), block, 0, st, pc);
-        else hipLaunchKernelGGL((k_far<double, P, 2, N64b>), fg(N64b), block, 0, st, pc);
+        constexpr int L32 = MHS_FAR_LPP32, L64 = MHS_FAR_LPP64, L32b = MHS_FAR_LPP32B, L64b = MHS_FAR_LPP64B;
+        if (nout == 1 && mixed) hipLaunchKernelGGL((k_far<float, P, 1, N32, L32>), fg(L32), block, 0, st, pc);
+        else if (nout == 1) hipLaunchKernelGGL((k_far<double, P, 1, N64, L64>), fg(L64), block, 0, st, pc);
+        else if (mixed) hipLaunchKernelGGL((k_far<float, P, 2, N32b, L32b>), fg(L32b), block, 0, st, pc);
+        else hipLaunchKernelGGL((k_far<double, P, 2, N64b, L64b>), fg(L64b), block, 0, st, pc);
         if (check(hipGetLastError(), "k_far launch")) return -1;
       }
       if (stages & MHS_STAGE_PAIR) {
