@@ -50,6 +50,16 @@
 #ifndef MHS_FAR_WAVES64
 #define MHS_FAR_WAVES64 2 // occupancy target of the far pass, FP64 points
 #endif
+// Occupancy targets of the closed-form kernels, whose per-item work is a
+// called function: 2 waves per SIMD beat 1 (latency of the long FP64 chains)
+// and 3 or 4 (spills), measured at n = 10^4: fold near points 47 -> 37 ms,
+// near pass 18 -> 12.7 ms.
+#ifndef MHS_FOLD_NEAR_WAVES
+#define MHS_FOLD_NEAR_WAVES 2  // the fold's near-point kernel
+#endif
+#ifndef MHS_NEAR_WAVES
+#define MHS_NEAR_WAVES 2       // the near pass
+#endif
 #ifndef MHS_FOLD_FAR_WAVES
 #define MHS_FOLD_FAR_WAVES 2   // occupancy target of the fold's far-point kernel
 #endif
@@ -512,7 +522,7 @@ __device__ __attribute__((noinline)) void near_item_call(const Params& pr, int64
 }
 
 template <int P>
-__global__ void __launch_bounds__(256) k_near(Params pr) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MHS_NEAR_WAVES))) k_near(Params pr) {
   // a block is one receiver row x MHS_NEAR_TILE columns; about one pair in
   // nine has a near part, so the block first lists its pairs that do (those
   // with an image or mirror part first, then the direct-only ones) and then
@@ -620,6 +630,14 @@ __global__ void __launch_bounds__(64) k_fold_prep(Params pr, const int64_t* __re
   }
 }
 
+// The closed-form surface load as a function of its own: inlined into the
+// point loop it spills more and runs slower.
+template <int P>
+__device__ __attribute__((noinline)) void surface_load_call(const Params& pr, const double* __restrict__ src,
+                                                            int k, double x, double y, double* sl) {
+  surface_load<P>(pr, src, k, x, y, sl);
+}
+
 // The Boussinesq sum of the folded pairs over their polar grids, split by
 // point: NEAR the points within the closed-form range of the source
 // (sqrt(|x - c|^2 + eps^2) <= 10 L) plus Love's disk term, FAR the others by
@@ -631,7 +649,8 @@ __global__ void __launch_bounds__(64) k_fold_prep(Params pr, const int64_t* __re
 // near, and without the list most far-kernel lanes would idle.
 #define MHS_FOLD_LIST 4096
 template <int P, bool STRESS, bool NEAR>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEAR ? 1 : MHS_FOLD_FAR_WAVES)))
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(NEAR ? MHS_FOLD_NEAR_WAVES : MHS_FOLD_FAR_WAVES)))
 k_fold(Params pr, const int64_t* __restrict__ list,
                                               const int* __restrict__ count, int64_t cap,
                                               const FoldItem* __restrict__ work, int64_t work_items) {
@@ -707,7 +726,7 @@ k_fold(Params pr, const int64_t* __restrict__ list,
           double x, y, w;
           fold_grid_point(G, xh0, xh1, u6, w6, u4, w4, g, x, y, w);
           double sl[3];
-          if (NEAR) surface_load<P>(pr, src, k, x, y, sl);
+          if (NEAR) surface_load_call<P>(pr, src, k, x, y, sl);
           else surface_load_far<P>(pr, src, k, x, y, sl);
           sl[0] -= s0; sl[1] -= s1; sl[2] -= s2;
           const double dx = xh0 - x, dy = xh1 - y;
