@@ -60,6 +60,9 @@
 #ifndef MHS_NEAR_WAVES
 #define MHS_NEAR_WAVES 2       // the near pass
 #endif
+#ifndef MHS_PAIR_WAVES
+#define MHS_PAIR_WAVES 2       // the pair pass of the displacement form
+#endif
 #ifndef MHS_FOLD_FAR_WAVES
 #define MHS_FOLD_FAR_WAVES 2   // occupancy target of the fold's far-point kernel
 #endif
@@ -90,12 +93,86 @@ static unsigned grid_1d(int64_t n) {
 }
 
 // ------------------------------------------------------------------ pair --
-template <int P, int MODE>   // MODE 0: stress family, 1: displacement, 2: eigenstress only
-__global__ void __launch_bounds__(256) k_pair(Params pr) {
-  const int64_t o = pr.obs_lo + blockIdx.y;
-  const int64_t col = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (col >= pr.n_cols) return;
+// Row tiles of the pair kernels: a block (256 threads) takes one receiver
+// row and MHS_ROW_TILE columns and sorts the columns into two LDS lists by
+// cls(col) (1: the first list, from list[0]; 2: the second, from
+// list[MHS_ROW_TILE]; 0: neither), each in ascending column order, by wave
+// ballots; the lists then run with every lane busy and without mixing the
+// two kinds of work in a wave.
+#define MHS_ROW_TILE 2048
+
+template <typename Cls>
+__device__ __forceinline__ void list_row_tile(const Params& pr, int64_t c0, Cls cls, int* list, int& n1, int& n2) {
+  __shared__ int cnt_s[2][4];
+  const int tid = threadIdx.x, wave = tid >> 6;
+  n1 = n2 = 0;
+  for (int step = 0; step < MHS_ROW_TILE; step += 256) {
+    const int64_t col = c0 + step + tid;
+    const int c = col < pr.n_cols ? cls(col) : 0;
+    const unsigned long long m1 = __ballot(c == 1), m2 = __ballot(c == 2);
+    if ((tid & 63) == 0) {
+      cnt_s[0][wave] = __popcll(m1);
+      cnt_s[1][wave] = __popcll(m2);
+    }
+    __syncthreads();
+    int o1 = n1, o2 = n2, s1 = 0, s2 = 0;
+#pragma unroll
+    for (int wv = 0; wv < 4; ++wv) {
+      o1 += wv < wave ? cnt_s[0][wv] : 0;
+      o2 += wv < wave ? cnt_s[1][wv] : 0;
+      s1 += cnt_s[0][wv];
+      s2 += cnt_s[1][wv];
+    }
+    if (c == 1)
+      list[o1 + __builtin_amdgcn_mbcnt_hi((unsigned)(m1 >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m1, 0u))] =
+          step + tid;
+    if (c == 2)
+      list[MHS_ROW_TILE + o2 +
+           __builtin_amdgcn_mbcnt_hi((unsigned)(m2 >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m2, 0u))] =
+          step + tid;
+    n1 += s1;
+    n2 += s2;
+    __syncthreads();          // cnt_s is rewritten by the next step
+    if (c0 + step + 256 >= pr.n_cols) break;
+  }
+}
+
+// One pair of the uncontracted forms as a function of its own (inlined into
+// the list loop its closed forms spill more).
+template <int P, int MODE>
+__device__ __attribute__((noinline)) void pair_item_call(const Params& pr, int64_t o, int64_t col) {
   pair_item<P, MODE>(pr, o, col);
+}
+
+// The pair pass of the uncontracted forms.  Displacement: a row tile's pairs
+// that are far on every part (Gauss tables only) first, then those with a
+// closed-form part (direct near, image near, or the folded mirror), at two
+// waves per SIMD.  The stress family keeps a thread per pair at one wave per
+// SIMD: measured at n = 2000 (stress, traction, 3 x 3 interaction) the lists
+// and the higher occupancy were 5-10 % slower for it, and 10-25 % faster
+// for displacement.
+template <int P, int MODE>   // MODE 0: stress family, 1: displacement, 2: eigenstress only
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == 1 ? MHS_PAIR_WAVES : 1)))
+k_pair(Params pr) {
+  constexpr int K = (P + 1) * (P + 2) / 2;
+  if (MODE != 1) {
+    const int64_t col = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (col < pr.n_cols) pair_item<P, MODE>(pr, pr.obs_lo + blockIdx.y, col);
+    return;
+  }
+  __shared__ int list_s[2 * MHS_ROW_TILE];
+  const int64_t o = pr.obs_lo + blockIdx.y;
+  const int64_t c0 = (int64_t)blockIdx.x * MHS_ROW_TILE;
+  const double ob[3] = {pr.obs[3 * o], pr.obs[3 * o + 1], pr.obs[3 * o + 2]};
+  int n_far, n_near;
+  list_row_tile(pr, c0, [&](int64_t col) {
+    return near_parts(pr, pr.src + (col / K) * MHS_SRC_STRIDE, ob) ? 2 : 1;
+  }, list_s, n_far, n_near);
+  for (int i = threadIdx.x; i < n_far + n_near; i += 256) {
+    __asm__ volatile("" ::: "memory");   // no loads hoisted out of the pair loop
+    const int c = i < n_far ? list_s[i] : list_s[MHS_ROW_TILE + i - n_far];
+    pair_item_call<P, MODE>(pr, o, c0 + c);
+  }
 }
 
 // --------------------------------------------------------------- Q-family --
@@ -512,7 +589,6 @@ k_far(Params pr) {
 
 // Near parts of the contracted interaction (closed forms, folded mirror),
 // added after k_far.
-#define MHS_NEAR_TILE 2048
 
 // One pair's near parts as a real call: inlined into the list loop its
 // closed forms spill far more than as a function of its own.
@@ -521,60 +597,29 @@ __device__ __attribute__((noinline)) void near_item_call(const Params& pr, int64
   pair_near_item<P>(pr, o, col);
 }
 
+// A block is one receiver row x MHS_ROW_TILE columns; about one pair in nine
+// has a near part, so the block lists its pairs that do (those with an image
+// or mirror part first, then the direct-only ones) and evaluates the list
+// with every lane busy; each pair is one thread's.
 template <int P>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MHS_NEAR_WAVES))) k_near(Params pr) {
-  // a block is one receiver row x MHS_NEAR_TILE columns; about one pair in
-  // nine has a near part, so the block first lists its pairs that do (those
-  // with an image or mirror part first, then the direct-only ones) and then
-  // evaluates the list with every lane busy; each pair is one thread's
   constexpr int K = (P + 1) * (P + 2) / 2;
-  __shared__ int list_s[2 * MHS_NEAR_TILE];   // image-part pairs at 0.., direct-only at MHS_NEAR_TILE..
-  __shared__ int cnt_s[2][4];
-  const int tid = threadIdx.x, wave = tid >> 6;
+  __shared__ int list_s[2 * MHS_ROW_TILE];
   const int64_t o = pr.obs_lo + blockIdx.y;
-  const int64_t c0 = (int64_t)blockIdx.x * MHS_NEAR_TILE;
+  const int64_t c0 = (int64_t)blockIdx.x * MHS_ROW_TILE;
   const double ob[3] = {pr.obs[3 * o], pr.obs[3 * o + 1], pr.obs[3 * o + 2]};
-  int n_img = 0, n_dir = 0;
-  for (int step = 0; step < MHS_NEAR_TILE; step += 256) {
-    const int64_t col = c0 + step + tid;
-    const int parts = col < pr.n_cols ? near_parts(pr, pr.src + (col / K) * MHS_SRC_STRIDE, ob) : 0;
-    const bool img = (parts & 2) != 0, dir = parts == 1;
-    const unsigned long long mi = __ballot(img), md = __ballot(dir);
-    if ((tid & 63) == 0) {
-      cnt_s[0][wave] = __popcll(mi);
-      cnt_s[1][wave] = __popcll(md);
-    }
-    __syncthreads();
-    int oi = n_img, od = n_dir, si = 0, sd = 0;
-#pragma unroll
-    for (int wv = 0; wv < 4; ++wv) {
-      oi += wv < wave ? cnt_s[0][wv] : 0;
-      od += wv < wave ? cnt_s[1][wv] : 0;
-      si += cnt_s[0][wv];
-      sd += cnt_s[1][wv];
-    }
-    if (img)
-      list_s[oi + __builtin_amdgcn_mbcnt_hi((unsigned)(mi >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mi, 0u))] =
-          step + tid;
-    if (dir)
-      list_s[MHS_NEAR_TILE + od +
-             __builtin_amdgcn_mbcnt_hi((unsigned)(md >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)md, 0u))] =
-          step + tid;
-    n_img += si;
-    n_dir += sd;
-    __syncthreads();          // cnt_s is rewritten by the next step
-    if (c0 + step + 256 >= pr.n_cols) break;
-  }
-  for (int i = tid; i < n_img + n_dir; i += 256) {
+  int n_img, n_dir;
+  list_row_tile(pr, c0, [&](int64_t col) {
+    const int parts = near_parts(pr, pr.src + (col / K) * MHS_SRC_STRIDE, ob);
+    return (parts & 2) ? 1 : parts == 1 ? 2 : 0;
+  }, list_s, n_img, n_dir);
+  for (int i = threadIdx.x; i < n_img + n_dir; i += 256) {
     __asm__ volatile("" ::: "memory");   // no loads hoisted out of the pair loop
-    const int c = i < n_img ? list_s[i] : list_s[MHS_NEAR_TILE + i - n_img];
+    const int c = i < n_img ? list_s[i] : list_s[MHS_ROW_TILE + i - n_img];
     near_item_call<P>(pr, o, c0 + c);
   }
 }
 
-// ------------------------------------------------------------------ fold --
-// Zero the fold-list counter (a fill kernel rather than a memset node, so the
-// launch sequence is safe to capture in a graph).
 __global__ void k_zero_i32(int* p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -910,6 +955,7 @@ static int launch_all(const mhs_launch* L, const Params& pr, hipStream_t st) {
     Params pc = pr;
     pc.obs_lo = lo;
     const dim3 grid((unsigned)nblk, (unsigned)rows);
+    const dim3 tiles((unsigned)((pr.n_cols + MHS_ROW_TILE - 1) / MHS_ROW_TILE), (unsigned)rows);
     if (contracted) {
       // far pass (writes every entry), then the near pass (adds)
       if (stages & MHS_STAGE_Q) {
@@ -925,14 +971,13 @@ static int launch_all(const mhs_launch* L, const Params& pr, hipStream_t st) {
         if (check(hipGetLastError(), "k_far launch")) return -1;
       }
       if (stages & MHS_STAGE_PAIR) {
-        const dim3 ng((unsigned)((pr.n_cols + MHS_NEAR_TILE - 1) / MHS_NEAR_TILE), (unsigned)rows);
-        hipLaunchKernelGGL((k_near<P>), ng, block, 0, st, pc);
+        hipLaunchKernelGGL((k_near<P>), tiles, block, 0, st, pc);
         if (check(hipGetLastError(), "k_near launch")) return -1;
       }
       continue;
     }
     if (stages & MHS_STAGE_PAIR) {
-      if (pr.kind == K_DISP) hipLaunchKernelGGL((k_pair<P, 1>), grid, block, 0, st, pc);
+      if (pr.kind == K_DISP) hipLaunchKernelGGL((k_pair<P, 1>), tiles, block, 0, st, pc);
       else if (pr.kind == K_EIGEN) hipLaunchKernelGGL((k_pair<P, 2>), grid, block, 0, st, pc);
       else hipLaunchKernelGGL((k_pair<P, 0>), grid, block, 0, st, pc);
       if (check(hipGetLastError(), "k_pair launch")) return -1;
