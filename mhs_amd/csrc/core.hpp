@@ -625,17 +625,26 @@ MHS_INLINE double centroid_distance(const double* c, const double o[3], double e
 
 // Weighted tables of node k by the hybrid switch (moments.py:551-601);
 // returns the plane height z.
+// Coordinates of o in the frame f: returns the normal offset z; X1, X2 in
+// the plane, h2 = z^2 + eps^2.
+MHS_INLINE double frame_local(const Frame& f, const double o[3], double eps, double& X1, double& X2,
+                              double& h2) {
+  const double d0 = o[0] - f.c[0], d1 = o[1] - f.c[1], d2 = o[2] - f.c[2];
+  const double z = d0 * f.n[0] + d1 * f.n[1] + d2 * f.n[2];
+  X1 = d0 * f.e1[0] + d1 * f.e1[1] + d2 * f.e1[2];
+  X2 = d0 * f.e2[0] + d1 * f.e2[1] + d2 * f.e2[2];
+  h2 = z * z + eps * eps;
+  return z;
+}
+
 template <class S, int P>
 MHS_INLINE double weighted_tables(const Frame& f, const double* __restrict__ v, int k,
                                   const double o[3], double eps,
                                   const double* __restrict__ gl_x, const double* __restrict__ gl_w,
                                   double W[6][21], double& X1o, double& X2o) {
-  const double d0 = o[0] - f.c[0], d1 = o[1] - f.c[1], d2 = o[2] - f.c[2];
-  const double z = d0 * f.n[0] + d1 * f.n[1] + d2 * f.n[2];
-  const double X1 = d0 * f.e1[0] + d1 * f.e1[1] + d2 * f.e1[2];
-  const double X2 = d0 * f.e2[0] + d1 * f.e2[1] + d2 * f.e2[2];
+  double X1, X2, h2;
+  const double z = frame_local(f, o, eps, X1, X2, h2);
   X1o = X1; X2o = X2;
-  const double h2 = z * z + eps * eps;
   const double Dn = centroid_distance(f.c, o, eps);
   if (Dn <= 10.0 * f.L) {
     double M[6][36];

@@ -94,21 +94,21 @@ static unsigned grid_1d(int64_t n) {
 
 // ------------------------------------------------------------------ pair --
 // Row tiles of the pair kernels: a block (256 threads) takes one receiver
-// row and MHS_ROW_TILE columns and sorts the columns into two LDS lists by
-// cls(col) (1: the first list, from list[0]; 2: the second, from
-// list[MHS_ROW_TILE]; 0: neither), each in ascending column order, by wave
-// ballots; the lists then run with every lane busy and without mixing the
-// two kinds of work in a wave.
+// row and MHS_ROW_TILE items of it (columns, or sources) from c0 on, of n in
+// all, and sorts them into two LDS lists by cls(item) (1: the first list,
+// from list[0]; 2: the second, from list[MHS_ROW_TILE]; 0: neither), each in
+// ascending order, by wave ballots; the lists then run with every lane busy
+// and without mixing the two kinds of work in a wave.
 #define MHS_ROW_TILE 2048
 
 template <typename Cls>
-__device__ __forceinline__ void list_row_tile(const Params& pr, int64_t c0, Cls cls, int* list, int& n1, int& n2) {
+__device__ __forceinline__ void list_row_tile(int64_t n, int64_t c0, Cls cls, int* list, int& n1, int& n2) {
   __shared__ int cnt_s[2][4];
   const int tid = threadIdx.x, wave = tid >> 6;
   n1 = n2 = 0;
   for (int step = 0; step < MHS_ROW_TILE; step += 256) {
     const int64_t col = c0 + step + tid;
-    const int c = col < pr.n_cols ? cls(col) : 0;
+    const int c = col < n ? cls(col) : 0;
     const unsigned long long m1 = __ballot(c == 1), m2 = __ballot(c == 2);
     if ((tid & 63) == 0) {
       cnt_s[0][wave] = __popcll(m1);
@@ -133,7 +133,7 @@ __device__ __forceinline__ void list_row_tile(const Params& pr, int64_t c0, Cls 
     n1 += s1;
     n2 += s2;
     __syncthreads();          // cnt_s is rewritten by the next step
-    if (c0 + step + 256 >= pr.n_cols) break;
+    if (c0 + step + 256 >= n) break;
   }
 }
 
@@ -165,7 +165,7 @@ k_pair(Params pr) {
   const int64_t c0 = (int64_t)blockIdx.x * MHS_ROW_TILE;
   const double ob[3] = {pr.obs[3 * o], pr.obs[3 * o + 1], pr.obs[3 * o + 2]};
   int n_far, n_near;
-  list_row_tile(pr, c0, [&](int64_t col) {
+  list_row_tile(pr.n_cols, c0, [&](int64_t col) {
     return near_parts(pr, pr.src + (col / K) * MHS_SRC_STRIDE, ob) ? 2 : 1;
   }, list_s, n_far, n_near);
   for (int i = threadIdx.x; i < n_far + n_near; i += 256) {
@@ -317,14 +317,17 @@ __device__ __forceinline__ void radii(double R2, double d3, bool q, double& ir, 
 
 // One quadrature pass of a pair over the ng x ng collapsed Gauss points of
 // the source triangle (IMG false: d = x - y) or of its mirror (IMG true:
-// d = x - y*), summing the selected series into acc_d (direct) / acc_i.
-// LPP lanes share the points, NPT per lane per chunk of LPP * NPT.  The
-// point comes from the flattened rule table; the offset is
+// d = x - y*), summing the selected series, times wscale, into acc[k][t] for
+// every node k of the source: the series of a point does not depend on the
+// node, only its weight N_k does, so a point is evaluated once for all K
+// nodes.  LPP lanes share the points, NPT per lane per chunk of LPP * NPT.
+// The point comes from the flattened rule table; the offset is
 // d = (x - v0) - x1 (v1 - v0) - x2 (v2 - v0) in FP64, mirrored in z for IMG.
 template <typename T, int P, int NOUT, int NPT, int LPP, bool IMG, bool DIR, bool RF, bool QF>
-__device__ __forceinline__ void far_pass(const Params& pr, const double* __restrict__ src, int k,
+__device__ __forceinline__ void far_pass(const Params& pr, const double* __restrict__ src,
                                          const double ob[3], int ng, int li, const T* kd, const T* kr,
-                                         const T* kq, double* acc_d, double* acc_i) {
+                                         const T* kq, double wscale, double (*acc)[NOUT]) {
+  constexpr int K = (P + 1) * (P + 2) / 2;
   constexpr int CH = LPP * NPT;
   const double* v = src + S_V;
   const double sz = IMG ? -1.0 : 1.0;           // z of the (mirrored) vertices
@@ -332,7 +335,7 @@ __device__ __forceinline__ void far_pass(const Params& pr, const double* __restr
   const double e1[3] = {v[3] - v[0], v[4] - v[1], sz * (v[5] - v[2])};
   const double e2[3] = {v[6] - v[0], v[7] - v[1], sz * (v[8] - v[2])};
   const T eps2 = (T)(src[S_EPS] * src[S_EPS]);
-  const double ar2 = 2.0 * src[S_AREA];
+  const double ar2 = 2.0 * src[S_AREA] * wscale;
   const size_t toff = (size_t)(ng - 1) * ng * (2 * ng - 1) / 6;
   const double* qx1 = pr.gq + toff;
   const double* qx2 = qx1 + MHS_GQ_TOTAL;
@@ -344,41 +347,50 @@ __device__ __forceinline__ void far_pass(const Params& pr, const double* __restr
     __asm__ volatile("" ::: "memory");
     T x[NPT], y[NPT], z[NPT], ir[NPT], iq[NPT], sv[NOUT * NPT];
     double wn[NPT];
+    T b1[NPT], b2[NPT];                         // barycentric point (P > 0: node weights)
 #pragma unroll
     for (int u = 0; u < NPT; ++u) {
       const int g0 = base + u * LPP + li;
       const bool in = g0 < npts;
       const int g = in ? g0 : 0;
       const double x1 = qx1[g], x2 = qx2[g];
-      wn[u] = in ? qw[g] * ar2 * shape_at<P>(k, 1.0 - x1 - x2, x1, x2) : 0.0;
+      wn[u] = in ? qw[g] * ar2 : 0.0;
+      b1[u] = (T)x1;
+      b2[u] = (T)x2;
       const T d1 = (T)fma(-x1, e1[0], fma(-x2, e2[0], ov[0]));
       const T d2 = (T)fma(-x1, e1[1], fma(-x2, e2[1], ov[1]));
       const T d3 = (T)fma(-x1, e1[2], fma(-x2, e2[2], ov[2]));
       x[u] = d1; y[u] = d2; z[u] = d3;
       radii(d1 * d1 + d2 * d2 + d3 * d3 + eps2, d3, QF, ir[u], iq[u]);
     }
-    if (DIR) {
-      d_blocked<T, NPT, NOUT>(kd, x, y, z, ir, sv);
-#pragma unroll
-      for (int t = 0; t < NOUT; ++t)
-#pragma unroll
-        for (int u = 0; u < NPT; ++u) acc_d[t] = fma(wn[u], (double)sv[t * NPT + u], acc_d[t]);
-    }
+    if (DIR) d_blocked<T, NPT, NOUT>(kd, x, y, z, ir, sv);
     if (RF) {
       __asm__ volatile("" ::: "memory");
       r_blocked<T, NPT, NOUT>(kr, x, y, z, ir, iq, sv);
-#pragma unroll
-      for (int t = 0; t < NOUT; ++t)
-#pragma unroll
-        for (int u = 0; u < NPT; ++u) acc_i[t] = fma(wn[u], (double)sv[t * NPT + u], acc_i[t]);
     }
     if (QF) {
       __asm__ volatile("" ::: "memory");
       q_blocked<T, NPT, NOUT>(kq, x, y, z, ir, iq, sv);
+    }
 #pragma unroll
-      for (int t = 0; t < NOUT; ++t)
+    for (int u = 0; u < NPT; ++u) {
 #pragma unroll
-        for (int u = 0; u < NPT; ++u) acc_i[t] = fma(wn[u], (double)sv[t * NPT + u], acc_i[t]);
+      for (int t = 0; t < NOUT; ++t) {
+        if (P == 0) {
+          acc[0][t] = fma(wn[u], (double)sv[t * NPT + u], acc[0][t]);
+        } else {
+          const double val = wn[u] * (double)sv[t * NPT + u];
+          const T l1 = b1[u], l2 = b2[u], l0 = (T)1 - l1 - l2;
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            T nk;
+            if (P == 1) nk = k == 0 ? l0 : k == 1 ? l1 : l2;
+            else if (k < 3) { const T l = k == 0 ? l0 : k == 1 ? l1 : l2; nk = l * ((T)2 * l - (T)1); }
+            else nk = (T)4 * (k == 3 ? l0 * l1 : k == 4 ? l1 * l2 : l2 * l0);
+            acc[k][t] = fma((double)nk, val, acc[k][t]);
+          }
+        }
+      }
     }
   }
 }
@@ -397,7 +409,7 @@ k_far(Params pr) {
   __shared__ T kq_s[PPB][NOUT * MHS_QM];
   __shared__ T kr_s[PPB][NOUT * MHS_RM];
   __shared__ T kd_s[PPB][NOUT * MHS_DC];
-  __shared__ double res_s[PPB * NOUT];      // a tile's results, stored as one contiguous run
+  __shared__ double res_s[PPB * K * NOUT];  // a tile's results, stored as one contiguous run
   const int tid = threadIdx.x;
   const int n_a = pr.n_a, n_b = pr.n_b;
   const int64_t o = pr.obs_lo + blockIdx.y;
@@ -437,15 +449,13 @@ k_far(Params pr) {
     __asm__ volatile("" : "+v"(lane));
     lane &= 255u;
     const int grp = (int)(lane / LPP), li = (int)(lane % LPP);
-    const int64_t col = ((int64_t)blockIdx.x * pr.far_tiles + tile) * PPB + grp;
-    const bool valid = col < pr.n_cols;
-    int64_t s = 0;
-    int k = 0, nd = 0, ni = 0, nq = 0;
+    // a pair is (receiver row, source): its K node columns share every point
+    const int64_t s = ((int64_t)blockIdx.x * pr.far_tiles + tile) * PPB + grp;
+    const bool valid = s < pr.n_src;
+    int nd = 0, ni = 0, nq = 0;
     bool dfar = false, rfar = false, img = false;
     const double* src = pr.src;
     if (valid) {
-      s = col / K;
-      k = (int)(col - s * K);
       src = pr.src + s * MHS_SRC_STRIDE;
       const double eps = src[S_EPS];
       const double Dd = centroid_distance(src + S_C, ob, eps);
@@ -541,48 +551,50 @@ k_far(Params pr) {
       }
     }
     __syncthreads();
-    double acc_d[NOUT], acc_i[NOUT];
+    double acc[K][NOUT];
 #pragma unroll
-    for (int t = 0; t < NOUT; ++t) acc_d[t] = acc_i[t] = 0.0;
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int t = 0; t < NOUT; ++t) acc[k][t] = 0.0;
     const T* kd = kd_s[grp];
     const T* kr = kr_s[grp];
     const T* kq = kq_s[grp];
     // 12 x 12 rules in whole chunks: NPT points per lane where LPP * NPT
     // divides 144, else chunks of 48 (16 lanes x 3 or 8 lanes x 6)
     constexpr int N12 = 144 % (LPP * NPT) == 0 ? NPT : LPP == 16 ? 3 : LPP == 8 ? 6 : NPT;
+    // the direct series enters as -C1 times its sum
     if (valid && dfar) {
       if (nd == 12)
-        far_pass<T, P, NOUT, N12, LPP, false, true, false, false>(pr, src, k, ob, 12, li, kd, kr, kq, acc_d, acc_i);
+        far_pass<T, P, NOUT, N12, LPP, false, true, false, false>(pr, src, ob, 12, li, kd, kr, kq, -pr.C1, acc);
       else
-        far_pass<T, P, NOUT, NPT, LPP, false, true, false, false>(pr, src, k, ob, nd, li, kd, kr, kq, acc_d, acc_i);
+        far_pass<T, P, NOUT, NPT, LPP, false, true, false, false>(pr, src, ob, nd, li, kd, kr, kq, -pr.C1, acc);
     }
     // the R and Q series run as separate passes even at equal orders: summed
     // in one pass their registers no longer fit the occupancy target
     if (valid && img && rfar) {
       if (ni == 12)
-        far_pass<T, P, NOUT, N12, LPP, true, false, true, false>(pr, src, k, ob, 12, li, kd, kr, kq, acc_d, acc_i);
+        far_pass<T, P, NOUT, N12, LPP, true, false, true, false>(pr, src, ob, 12, li, kd, kr, kq, 1.0, acc);
       else
-        far_pass<T, P, NOUT, NPT, LPP, true, false, true, false>(pr, src, k, ob, ni, li, kd, kr, kq, acc_d, acc_i);
+        far_pass<T, P, NOUT, NPT, LPP, true, false, true, false>(pr, src, ob, ni, li, kd, kr, kq, 1.0, acc);
     }
     if (valid && img)
-      far_pass<T, P, NOUT, NPT, LPP, true, false, false, true>(pr, src, k, ob, nq, li, kd, kr, kq, acc_d, acc_i);
-    double res[NOUT];
+      far_pass<T, P, NOUT, NPT, LPP, true, false, false, true>(pr, src, ob, nq, li, kd, kr, kq, 1.0, acc);
 #pragma unroll
-    for (int t = 0; t < NOUT; ++t) {
-      res[t] = fma(-pr.C1, acc_d[t], acc_i[t]);
-      for (int off = LPP / 2; off > 0; off >>= 1) res[t] += __shfl_xor(res[t], off, LPP);
-    }
-    if (li == 0) {
+    for (int k = 0; k < K; ++k)
 #pragma unroll
-      for (int t = 0; t < NOUT; ++t) res_s[grp * NOUT + t] = res[t];
-    }
+      for (int t = 0; t < NOUT; ++t) {
+        double r = acc[k][t];
+        for (int off = LPP / 2; off > 0; off >>= 1) r += __shfl_xor(r, off, LPP);
+        if (li == 0) res_s[(grp * K + k) * NOUT + t] = r;
+      }
     __syncthreads();     // the kappas are rewritten by the next tile; res_s is complete
     {
-      // the tile's PPB consecutive columns are one contiguous run of the row
-      const int64_t col0 = ((int64_t)blockIdx.x * pr.far_tiles + tile) * PPB;
+      // the tile's PPB sources are PPB * K consecutive columns: one contiguous run of the row
+      const int64_t s0 = ((int64_t)blockIdx.x * pr.far_tiles + tile) * PPB;
       const int64_t oo = o - pr.out_lo;
-      const int64_t n_here = (pr.n_cols - col0 < PPB ? pr.n_cols - col0 : PPB) * NOUT;
-      if (tid < n_here) pr.out[(oo * pr.n_cols + col0) * NOUT + tid] = res_s[tid];
+      const int64_t n_here = (pr.n_src - s0 < PPB ? pr.n_src - s0 : PPB) * K * NOUT;
+      double* dst = pr.out + (oo * pr.n_cols + s0 * K) * NOUT;
+      for (int64_t i = tid; i < n_here; i += 256) dst[i] = res_s[i];
     }
   }
 }
@@ -593,30 +605,30 @@ k_far(Params pr) {
 // One pair's near parts as a real call: inlined into the list loop its
 // closed forms spill far more than as a function of its own.
 template <int P>
-__device__ __attribute__((noinline)) void near_item_call(const Params& pr, int64_t o, int64_t col) {
-  pair_near_item<P>(pr, o, col);
+__device__ __attribute__((noinline)) void near_src_call(const Params& pr, int64_t o, int64_t s) {
+  pair_near_src<P>(pr, o, s);
 }
 
-// A block is one receiver row x MHS_ROW_TILE columns; about one pair in nine
-// has a near part, so the block lists its pairs that do (those with an image
-// or mirror part first, then the direct-only ones) and evaluates the list
-// with every lane busy; each pair is one thread's.
+// A block is one receiver row x MHS_ROW_TILE sources; about one pair in nine
+// has a near part, so the block lists its sources that do (those with an
+// image or mirror part first, then the direct-only ones) and evaluates the
+// list with every lane busy; each (row, source) pair, all its node columns,
+// is one thread's.
 template <int P>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MHS_NEAR_WAVES))) k_near(Params pr) {
-  constexpr int K = (P + 1) * (P + 2) / 2;
   __shared__ int list_s[2 * MHS_ROW_TILE];
   const int64_t o = pr.obs_lo + blockIdx.y;
   const int64_t c0 = (int64_t)blockIdx.x * MHS_ROW_TILE;
   const double ob[3] = {pr.obs[3 * o], pr.obs[3 * o + 1], pr.obs[3 * o + 2]};
   int n_img, n_dir;
-  list_row_tile(pr, c0, [&](int64_t col) {
-    const int parts = near_parts(pr, pr.src + (col / K) * MHS_SRC_STRIDE, ob);
+  list_row_tile(pr.n_src, c0, [&](int64_t s) {
+    const int parts = near_parts(pr, pr.src + s * MHS_SRC_STRIDE, ob);
     return (parts & 2) ? 1 : parts == 1 ? 2 : 0;
   }, list_s, n_img, n_dir);
   for (int i = threadIdx.x; i < n_img + n_dir; i += 256) {
     __asm__ volatile("" ::: "memory");   // no loads hoisted out of the pair loop
     const int c = i < n_img ? list_s[i] : list_s[MHS_ROW_TILE + i - n_img];
-    near_item_call<P>(pr, o, c0 + c);
+    near_src_call<P>(pr, o, c0 + c);
   }
 }
 
@@ -959,9 +971,9 @@ static int launch_all(const mhs_launch* L, const Params& pr, hipStream_t st) {
     if (contracted) {
       // far pass (writes every entry), then the near pass (adds)
       if (stages & MHS_STAGE_Q) {
-        auto fg = [&](int lpp) {   // pairs per block = 256 / lanes per pair
+        auto fg = [&](int lpp) {   // sources per block = 256 / lanes per pair, per tile
           const int64_t per = (int64_t)(256 / lpp) * pr.far_tiles;
-          return dim3((unsigned)((pr.n_cols + per - 1) / per), (unsigned)rows);
+          return dim3((unsigned)((pr.n_src + per - 1) / per), (unsigned)rows);
         };
         constexpr int L32 = MHS_FAR_LPP32, L64 = MHS_FAR_LPP64, L32b = MHS_FAR_LPP32B, L64b = MHS_FAR_LPP64B;
         if (nout == 1 && mixed) hipLaunchKernelGGL((k_far<float, P, 1, N32, L32>), fg(L32), block, 0, st, pc);
@@ -971,7 +983,8 @@ static int launch_all(const mhs_launch* L, const Params& pr, hipStream_t st) {
         if (check(hipGetLastError(), "k_far launch")) return -1;
       }
       if (stages & MHS_STAGE_PAIR) {
-        hipLaunchKernelGGL((k_near<P>), tiles, block, 0, st, pc);
+        const dim3 src_tiles((unsigned)((pr.n_src + MHS_ROW_TILE - 1) / MHS_ROW_TILE), (unsigned)rows);
+        hipLaunchKernelGGL((k_near<P>), src_tiles, block, 0, st, pc);
         if (check(hipGetLastError(), "k_near launch")) return -1;
       }
       continue;

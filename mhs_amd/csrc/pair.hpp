@@ -327,7 +327,7 @@ MHS_INLINE void pair_item(const Params& pr, int64_t o, int64_t col) {
 }
 
 // Which near parts a pair of the contracted interaction has, by the same
-// tests as pair_near_item: bit 0 the closed-form direct term, bit 1 the
+// tests as pair_near_src: bit 0 the closed-form direct term, bit 1 the
 // folded mirror (band) or the closed-form image R-family.
 MHS_INLINE int near_parts(const Params& pr, const double* __restrict__ src, const double ob[3]) {
   const double eps = src[S_EPS];
@@ -337,17 +337,14 @@ MHS_INLINE int near_parts(const Params& pr, const double* __restrict__ src, cons
   return (dnear ? 1 : 0) | ((band || rnear) ? 2 : 0);
 }
 
-// Near part of one (observer row o, element column col) of a contracted
-// interaction, added after the far pass: the closed-form direct term and its
-// eigenstress when the pair is near (sqrt(|x - c|^2 + eps^2) <= 10 L), and on
-// the image side either the folded mirror element (fold band, with its
-// eigenstress) or the closed-form R-family when the mirrored triangle is
-// near.  Everything far is the far pass's.
+// The near parts of the contracted interaction for every node column of
+// (receiver o, source s): the closed-form direct term, the folded mirror
+// element (band), or the closed-form image R-family, each added to the
+// column's entry.  The closed-form raw moments depend on the pair only, so
+// they are computed once per part and weighted per node.
 template <int P>
-MHS_INLINE void pair_near_item(const Params& pr, int64_t o, int64_t col) {
+MHS_INLINE void pair_near_src(const Params& pr, int64_t o, int64_t s) {
   constexpr int K = (P + 1) * (P + 2) / 2;
-  const int64_t s = col / K;
-  const int k = (int)(col - s * K);
   const double* src = pr.src + s * MHS_SRC_STRIDE;
   const double ob[3] = {pr.obs[3 * o], pr.obs[3 * o + 1], pr.obs[3 * o + 2]};
   const double eps = src[S_EPS];
@@ -381,16 +378,24 @@ MHS_INLINE void pair_near_item(const Params& pr, int64_t o, int64_t col) {
       rG[a][b] = g;
     }
   }
-  double acc[2] = {0.0, 0.0};
+  double acc[K][2];
+  for (int k = 0; k < K; ++k) acc[k][0] = acc[k][1] = 0.0;
+  double M[6][36];
   double W[6][21];
-  double X1, X2;
+  double c[6];
   if (dnear) {
     Frame f;
     load_frame(src, false, f);
-    const double z = weighted_tables<SpecH<P>, P>(f, src + S_V, k, ob, eps, pr.gl_x, pr.gl_w, W, X1, X2);
-    for (int a = 0; a < n_a; ++a)
-      for (int b = 0; b < n_b; ++b)
-        acc[a * n_b + b] += direct_contract(f, W, z, eps, mt, Ea[a], Fb[b], sub, rF[a][b]);
+    double X1, X2, h2;
+    const double z = frame_local(f, ob, eps, X1, X2, h2);
+    raw_moments_near<SpecH<P>>(f, src + S_V, X1, X2, h2, M);
+    for (int k = 0; k < K; ++k) {
+      shape_coeffs<P>(f, k, X1, X2, c);
+      weight_from_raw<SpecH<P>, P>(M, c, W);
+      for (int a = 0; a < n_a; ++a)
+        for (int b = 0; b < n_b; ++b)
+          acc[k][a * n_b + b] += direct_contract(f, W, z, eps, mt, Ea[a], Fb[b], sub, rF[a][b]);
+    }
   }
   if (band || rnear) {
     Frame g;
@@ -398,34 +403,48 @@ MHS_INLINE void pair_near_item(const Params& pr, int64_t o, int64_t col) {
     double vi[9];
     reflected_vertices(src, vi);
     if (band) {
-      const double z = weighted_tables<SpecH<P>, P>(g, vi, k, ob, eps, pr.gl_x, pr.gl_w, W, X1, X2);
-      for (int a = 0; a < n_a; ++a)
-        for (int b = 0; b < n_b; ++b)
-          acc[a * n_b + b] += direct_contract(g, W, z, eps, mt, Ea[a], Gb[b], sub, rG[a][b]);
+      // the mirror element, closed form or Gauss by its own distance (rare:
+      // the fold band), one node at a time
+      double X1, X2;
+      for (int k = 0; k < K; ++k) {
+        const double z = weighted_tables<SpecH<P>, P>(g, vi, k, ob, eps, pr.gl_x, pr.gl_w, W, X1, X2);
+        for (int a = 0; a < n_a; ++a)
+          for (int b = 0; b < n_b; ++b)
+            acc[k][a * n_b + b] += direct_contract(g, W, z, eps, mt, Ea[a], Gb[b], sub, rG[a][b]);
+      }
     } else {
-      const double zp = weighted_tables<SpecI<P>, P>(g, vi, k, ob, eps, pr.gl_x, pr.gl_w, W, X1, X2);
+      double X1, X2, h2;
+      const double zp = frame_local(g, ob, eps, X1, X2, h2);
+      raw_moments_near<SpecI<P>>(g, vi, X1, X2, h2, M);
       const double al[3] = {-g.e1[0], -g.e1[1], -g.e1[2]};
       const double be[3] = {-g.e2[0], -g.e2[1], -g.e2[2]};
       const double ga[3] = {zp * g.n[0], zp * g.n[1], zp * g.n[2]};
-      double DDG[81];
+      for (int k = 0; k < K; ++k) {
+        shape_coeffs<P>(g, k, X1, X2, c);
+        weight_from_raw<SpecI<P>, P>(M, c, W);
+        double DDG[81];
 #pragma unroll
-      for (int t = 0; t < 81; ++t) DDG[t] = 0.0;
-      r_ddg(W[2], W[3], W[4], W[5], al, be, ga, ob[2], pr.coef, DDG);
-      // rb.H.nr.sb of the R-family = sum over slots E[i][p] F[j][m] DDG[i][j][p][m]
-      const int ix[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
-      for (int a = 0; a < n_a; ++a)
-        for (int b = 0; b < n_b; ++b) {
-          double v = 0.0;
+        for (int t = 0; t < 81; ++t) DDG[t] = 0.0;
+        r_ddg(W[2], W[3], W[4], W[5], al, be, ga, ob[2], pr.coef, DDG);
+        // rb.H.nr.sb of the R-family = sum over slots E[i][p] F[j][m] DDG[i][j][p][m]
+        const int ix[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
+        for (int a = 0; a < n_a; ++a)
+          for (int b = 0; b < n_b; ++b) {
+            double v = 0.0;
 #pragma unroll
-          for (int t = 0; t < 81; ++t) {
-            const int i = t / 27, j = (t / 9) % 3, p = (t / 3) % 3, m = t % 3;
-            v = fma(Ea[a][ix[i][p]] * Fb[b][ix[j][m]], DDG[t], v);
+            for (int t = 0; t < 81; ++t) {
+              const int i = t / 27, j = (t / 9) % 3, p = (t / 3) % 3, m = t % 3;
+              v = fma(Ea[a][ix[i][p]] * Fb[b][ix[j][m]], DDG[t], v);
+            }
+            acc[k][a * n_b + b] += v;
           }
-          acc[a * n_b + b] += v;
-        }
+      }
     }
   }
   const int64_t oo = o - pr.out_lo;
-  for (int t = 0; t < n_a * n_b; ++t) pr.out[(oo * pr.n_cols + col) * (n_a * n_b) + t] += acc[t];
+  const int nout = n_a * n_b;
+  double* dst = pr.out + (oo * pr.n_cols + s * K) * nout;
+  for (int k = 0; k < K; ++k)
+    for (int t = 0; t < nout; ++t) dst[k * nout + t] += acc[k][t];
 }
 

@@ -130,6 +130,38 @@ def test_random_vs_oracle(M):
                                            source=("strike", "dip"))) < TOL
 
 
+def test_node_columns_share_points(M):
+    """The contracted far and near passes evaluate a (receiver, source) pair
+    once for all K node columns: one and two outputs at orders 1 and 2
+    against the FP64 oracle on a buried 4-triangle fault (near and far pairs,
+    no fold: the oracle's fold is slow), and the mixed mode (FP32 points and
+    node weights) against FP64 on a 240-triangle fault reaching the surface."""
+    from mhs_amd import workloads
+    from oracle import api as O
+    rng = np.random.default_rng(4321)
+    P = workloads.plane(1.6, 1.0, 2, 1, 70.0, -0.6, 0.0, 0.0, 30.0)
+    tris = workloads.orient_up(workloads.grid_mesh(P + rng.uniform(-0.03, 0.03, P.shape) * [1, 1, 0]))
+    tris[:, :, 2] = np.minimum(tris[:, :, 2], 0.0)
+    eps = rng.uniform(0.05, 0.15, tris.shape[0])
+    mat = M.Material(30.0, 20.0)
+    for p in (1, 2):
+        for rec, srcb in (("strike", "strike"), ("normal", ("strike", "dip"))):
+            print(f"case nodes p{p} {rec} {srcb}", flush=True)
+            assert relmax(M.interaction_matrix(tris, mat, eps, order=p, receiver=rec, source=srcb),
+                          O.interaction_matrix(tris, mat.mu, mat.lam, eps, order=p, receiver=rec,
+                                               source=srcb)) < TOL
+    Pb = workloads.plane(30.0, 8.0, 12, 10, 60.0, 0.0)
+    tb = workloads.orient_up(workloads.grid_mesh(Pb))
+    for p in (1, 2):
+        print(f"case nodes mixed p{p}", flush=True)
+        K64 = M.interaction_matrix(tb, mat, 0.1, order=p, receiver="strike", source="strike")
+        with M.precision("mixed"):
+            K32 = M.interaction_matrix(tb, mat, 0.1, order=p, receiver="strike", source="strike")
+        err = relmax(K32, K64)
+        print(f"mixed relmax vs fp64 p{p} {err:.3e}", flush=True)
+        assert err < 1e-6
+
+
 def test_identities_on_gpu(M, small):
     """total - eigenstress == elastic to rounding; interaction == stress
     contracted with the receiver plane and both frames (1e-13,
