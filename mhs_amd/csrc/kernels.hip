@@ -606,7 +606,10 @@ k_far(Params pr) {
 // closed forms spill far more than as a function of its own.
 template <int P>
 __device__ __attribute__((noinline)) void near_src_call(const Params& pr, int64_t o, int64_t s) {
-  pair_near_src<P>(pr, o, s);
+  // order 0: a source is one column, and the per-column form holds fewer
+  // registers and less scratch than the all-nodes form
+  if (P == 0) pair_near_item<P>(pr, o, s);
+  else pair_near_src<P>(pr, o, s);
 }
 
 // A block is one receiver row x MHS_ROW_TILE sources; about one pair in nine
