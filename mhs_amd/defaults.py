@@ -44,3 +44,8 @@ FOLD_R1_MARGIN_DIAM = 8.0
 
 # float32 out= storage tolerance (defaults.py:40-43)
 OUT_FLOAT32_PARITY = 1.0e-6
+
+# block drivers (parallel.py): the reference's in-flight budget of its worker
+# pool (defaults.py:82), and the result size of one device block here
+PARALLEL_INFLIGHT_BYTES = 512 << 20
+DEVICE_BLOCK_BYTES = 2 << 30

@@ -162,6 +162,36 @@ def test_node_columns_share_points(M):
         assert err < 1e-6
 
 
+def test_block_drivers_equal_one_call(M):
+    """The reference-compatible block drivers (parallel.by_source /
+    by_rows) stitch device blocks into bitwise the single call's matrix:
+    traction by source blocks, the contracted interaction by source blocks
+    with the receivers bound, and the continuous layout by row blocks."""
+    import functools
+    from mhs_amd import parallel as mp, workloads
+    rng = np.random.default_rng(99)
+    Pl = workloads.plane(12.0, 6.0, 8, 4, 55.0, 0.0)
+    tris = workloads.orient_up(workloads.grid_mesh(Pl + rng.uniform(-0.05, 0.05, Pl.shape) * [1, 1, 0]))
+    tris[:, :, 2] = np.minimum(tris[:, :, 2], 0.0)
+    mat = M.Material(30.0, 30.0)
+    obs = np.column_stack([rng.uniform(-8, 8, 50), rng.uniform(-8, 8, 50), -rng.uniform(0, 6, 50)])
+    nrm = rng.normal(size=(50, 3))
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    print("case by_source traction", flush=True)
+    call = functools.partial(M.traction_matrix, obs, nrm, material=mat, eps=0.1, order=1)
+    assert np.array_equal(mp.by_source(call, tris, block_bytes=200_000), call(tris))
+    print("case by_source interaction", flush=True)
+    call = functools.partial(M.interaction_matrix, material=mat, eps=0.1, obs_tris=tris,
+                             receiver="strike", source="strike")
+    assert np.array_equal(mp.by_source(call, tris, source_axis=1, tris_kw="tris", block_bytes=20_000),
+                          call(tris=tris))
+    print("case by_rows continuous", flush=True)
+    sn = M.shared_nodes(tris)
+    call = functools.partial(M.interaction_matrix, tris, mat, 0.1, order=2, layout="continuous", shared=sn)
+    full = call()
+    assert np.array_equal(mp.by_rows(call, full.shape[0], block_bytes=100_000), full)
+
+
 def test_identities_on_gpu(M, small):
     """total - eigenstress == elastic to rounding; interaction == stress
     contracted with the receiver plane and both frames (1e-13,
