@@ -110,6 +110,14 @@ int mhs_histogram(const mhs_launch* launch, unsigned long long* counts, void* st
 /* Elastic strain from elastic stress in place (matrices.py:572-579). */
 int mhs_strain(double* buf, int64_t n_obs, int64_t n_cols, double mu, double lam, void* stream);
 
+/* Source-major copy of an interaction matrix for consumers that read one
+ * source's column at every receiver (studies/active_fractal/kernel.py:14-23,
+ * 259-261, to_source_major): out[s][b][a][r] = in[r][s][a][b], in of shape
+ * (n_rec, n_src, n_a, n_b), out (n_src, n_b, n_a, n_rec), both contiguous
+ * device FP64; 1 <= n_a, n_b <= 3. */
+int mhs_source_major(const double* in, double* out, int64_t n_rec, int64_t n_src, int n_a, int n_b,
+                     void* stream);
+
 /* Per-material image-kernel coefficients into host memory (mhs_ncoef()
  * doubles): the exact rational image table with nu = lam / (2 (lam + mu)) and
  * the scale 2 (lam + mu) / (pi mu (lam + 2 mu)) folded in

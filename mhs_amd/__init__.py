@@ -10,7 +10,7 @@ C-ABI (``include/mhs_amd.h``).
     mat = mhs.Material(mu=30.0, lam=30.0)
     K = mhs.interaction_matrix(tris, mat, eps=0.1, receiver="strike", source="strike")
 """
-from . import chunking, defaults, parallel, tdcs
+from . import chunking, consumer, defaults, parallel, tdcs
 from .api import (collocation_points, disp_matrix, eigenstress_matrix, elastic_strain_matrix,
                   interaction_matrix, stress_matrix, total_stress_matrix, traction_matrix)
 from .materials import Material
@@ -20,7 +20,7 @@ from .shared import SharedNodes, shared_nodes
 __all__ = [
     "Material", "disp_matrix", "stress_matrix", "traction_matrix", "interaction_matrix",
     "total_stress_matrix", "eigenstress_matrix", "elastic_strain_matrix", "collocation_points",
-    "shared_nodes", "SharedNodes", "chunking", "defaults", "parallel", "tdcs", "precision",
+    "shared_nodes", "SharedNodes", "chunking", "consumer", "defaults", "parallel", "tdcs", "precision",
     "set_precision", "get_precision",
 ]
 

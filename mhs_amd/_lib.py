@@ -39,7 +39,7 @@ class Launch(C.Structure):
 _LIB = None
 
 EXPORTS = ("mhs_assemble", "mhs_histogram", "mhs_gather", "mhs_strain", "mhs_fill_material", "mhs_ncoef",
-           "mhs_src_stride", "mhs_abi_version", "mhs_last_error", "mhs_fold_item_bytes")
+           "mhs_src_stride", "mhs_abi_version", "mhs_last_error", "mhs_fold_item_bytes", "mhs_source_major")
 
 
 def load(path=None):
@@ -64,6 +64,8 @@ def load(path=None):
     lib.mhs_strain.restype = C.c_int
     lib.mhs_fill_material.argtypes = [C.c_double, C.c_double, C.c_void_p]
     lib.mhs_fill_material.restype = C.c_int
+    lib.mhs_source_major.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_int, C.c_int, C.c_void_p]
+    lib.mhs_source_major.restype = C.c_int
     lib.mhs_fold_item_bytes.argtypes = [C.c_int]
     lib.mhs_fold_item_bytes.restype = C.c_int64
     for nm in ("mhs_ncoef", "mhs_src_stride", "mhs_abi_version"):

@@ -930,6 +930,36 @@ __global__ void k_gather(const double* __restrict__ scr, double* __restrict__ ou
 
 // Elastic strain from stress (matrices.py:572-579): tr(e) = tr(sig)/(3 lam + 2 mu),
 // e = (sig - lam tr(e) I) / (2 mu), in place on (n_obs, 3, 3, n_cols, 3).
+// Source-major copy of an interaction matrix: out[s][b][a][r] = in[r][s][a][b]
+// (in n_rec x n_src x n_a x n_b, out n_src x n_b x n_a x n_rec).  A block moves
+// a tile of MHS_TR_R receivers x ts = MHS_TR_LD / (n_a n_b) sources through
+// LDS: each receiver's ts * n_a * n_b entries are one contiguous read, each
+// (source, b, a) row of the tile's receivers one contiguous write.  HBM-bound
+// (8 bytes read and 8 written per entry).
+#define MHS_TR_R 32
+#define MHS_TR_LD 64
+__global__ void __launch_bounds__(256) k_source_major(const double* __restrict__ in, double* __restrict__ out,
+                                                      int64_t n_rec, int64_t n_src, int n_a, int n_b) {
+  __shared__ double tile[MHS_TR_R][MHS_TR_LD + 1];
+  const int nab = n_a * n_b, ts = MHS_TR_LD / nab;
+  const int64_t r0 = (int64_t)blockIdx.y * MHS_TR_R, s0 = (int64_t)blockIdx.x * ts;
+  const int rows = (int)(n_rec - r0 < MHS_TR_R ? n_rec - r0 : MHS_TR_R);
+  const int cols = (int)(n_src - s0 < ts ? n_src - s0 : ts);
+  const int w = cols * nab;                     // contiguous entries per receiver in the tile
+  for (int i = threadIdx.x; i < rows * MHS_TR_LD; i += 256) {
+    const int r = i / MHS_TR_LD, c = i - r * MHS_TR_LD;
+    if (c < w) tile[r][c] = in[((r0 + r) * n_src + s0) * nab + c];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < cols * nab * MHS_TR_R; i += 256) {
+    const int q = i / MHS_TR_R, r = i - q * MHS_TR_R;   // q = (source, b, a) within the tile
+    if (r < rows) {
+      const int s = q / nab, ba = q - s * nab, b = ba / n_a, a = ba - b * n_a;
+      out[((s0 + s) * nab + ba) * n_rec + r0 + r] = tile[r][s * nab + a * n_b + b];
+    }
+  }
+}
+
 __global__ void k_strain(double* __restrict__ buf, int64_t n_obs, int64_t n_cols, double mu, double lam) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n_obs * n_cols * 3; t += stride) {
@@ -1118,6 +1148,20 @@ int mhs_strain(double* buf, int64_t n_obs, int64_t n_cols, double mu, double lam
   hipLaunchKernelGGL(k_strain, dim3(grid_1d(total)), dim3(256), 0, (hipStream_t)stream, buf,
                      n_obs, n_cols, mu, lam);
   return check(hipGetLastError(), "k_strain launch");
+}
+
+int mhs_source_major(const double* in, double* out, int64_t n_rec, int64_t n_src, int n_a, int n_b,
+                     void* stream) {
+  if (!in || !out) return fail("mhs_source_major: null device pointer");
+  if (n_a < 1 || n_a > 3 || n_b < 1 || n_b > 3) return fail("mhs_source_major: n_a, n_b must be in 1..3");
+  if (n_rec < 0 || n_src < 0) return fail("mhs_source_major: negative size");
+  if (n_rec == 0 || n_src == 0) return 0;
+  const int nab = n_a * n_b, ts = MHS_TR_LD / nab;
+  const int64_t gx = (n_src + ts - 1) / ts, gy = (n_rec + MHS_TR_R - 1) / MHS_TR_R;
+  if (gx * 256 > 0xffffffffLL || gy > 0x7fffffffLL) return fail("mhs_source_major: matrix too large for one launch");
+  hipLaunchKernelGGL(k_source_major, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, (hipStream_t)stream, in, out,
+                     n_rec, n_src, n_a, n_b);
+  return check(hipGetLastError(), "k_source_major launch");
 }
 
 int mhs_histogram(const mhs_launch* L, unsigned long long* counts, void* stream) {

@@ -1,0 +1,44 @@
+"""Content cache key of the consumer layout (keyed like
+studies/active_fractal/kernel.py:131-155): the same inputs give the same key,
+and anything the matrix depends on -- vertex order included -- changes it."""
+import numpy as np
+
+import mhs_amd as M
+from mhs_amd import consumer
+
+
+def _mesh():
+    rng = np.random.default_rng(3)
+    return rng.normal(size=(12, 3, 3)) - [0, 0, 5.0]
+
+
+def test_cache_key_is_stable_and_content_based():
+    tris, mat = _mesh(), M.Material(30.0, 30.0)
+    k = consumer.cache_key(tris, mat, 0.1, receiver="strike", source="strike")
+    assert len(k) == 16 and int(k, 16) >= 0
+    assert k == consumer.cache_key(tris.copy(), M.Material(30.0, 30.0), 0.1, source="strike", receiver="strike")
+    variants = [
+        consumer.cache_key(tris[::-1], mat, 0.1, receiver="strike", source="strike"),
+        consumer.cache_key(tris[:, ::-1], mat, 0.1, receiver="strike", source="strike"),
+        consumer.cache_key(tris, M.Material(30.0, 31.0), 0.1, receiver="strike", source="strike"),
+        consumer.cache_key(tris, mat, 0.11, receiver="strike", source="strike"),
+        consumer.cache_key(tris, mat, np.full(12, 0.1), receiver="strike", source="strike"),
+        consumer.cache_key(tris, mat, 0.1, receiver="dip", source="strike"),
+        consumer.cache_key(tris, mat, 0.1, receiver="strike", source="strike", order=1),
+        consumer.cache_key(tris, mat, 0.1, receiver="strike", source="strike", layout_tag="row_major"),
+    ]
+    assert len(set(variants + [k])) == len(variants) + 1
+
+
+def test_cache_key_per_source_eps_and_shared_nodes():
+    tris, mat = _mesh(), M.Material(30.0, 30.0)
+    e = np.linspace(0.05, 0.15, 12)
+    e2 = e.copy()
+    e2[3] += 1e-12
+    assert consumer.cache_key(tris, mat, e) != consumer.cache_key(tris, mat, e2)
+    from mhs_amd import workloads
+    grid = workloads.orient_up(workloads.grid_mesh(workloads.plane(4.0, 2.0, 2, 1, 60.0, -1.0)))
+    a = consumer.cache_key(grid, mat, 0.1, order=2, layout="continuous", shared=M.shared_nodes(grid))
+    b = consumer.cache_key(grid, mat, 0.1, order=2, layout="continuous", shared=M.shared_nodes(grid.copy()))
+    assert a == b
+    assert a != consumer.cache_key(grid, mat, 0.1, order=2, layout="continuous")

@@ -192,6 +192,35 @@ def test_block_drivers_equal_one_call(M):
     assert np.array_equal(mp.by_rows(call, full.shape[0], block_bytes=100_000), full)
 
 
+def test_source_major_layout(M):
+    """The consumer layout (n_src, n_b, n_a, n_rec) copied on the device is
+    bitwise the transpose of interaction_matrix, for 1, 2, 6 and 9
+    components, tiles with ragged edges, a row range and the continuous
+    layout; float32 out= and the device tensor too."""
+    import torch
+    from mhs_amd import consumer, workloads
+    rng = np.random.default_rng(17)
+    Pl = workloads.plane(10.0, 5.0, 9, 4, 65.0, -0.3)
+    tris = workloads.orient_up(workloads.grid_mesh(Pl + rng.uniform(-0.05, 0.05, Pl.shape) * [1, 1, 0]))
+    mat = M.Material(30.0, 25.0)
+    cases = [dict(receiver="strike", source="strike"), dict(receiver="dip", source=("strike", "dip")),
+             dict(receiver=("strike", "dip"), source=("strike", "dip", "tensile")), dict(),
+             dict(order=1, rows=(5, 40)), dict(order=2, layout="continuous", shared=M.shared_nodes(tris))]
+    for kw in cases:
+        print(f"case source-major {kw.get('receiver')} {kw.get('source')} {kw.get('order', 0)}", flush=True)
+        K = M.interaction_matrix(tris, mat, 0.1, **kw)
+        S = consumer.interaction_source_major(tris, mat, 0.1, **kw)
+        assert S.flags.c_contiguous and S.shape == (K.shape[1], K.shape[3], K.shape[2], K.shape[0])
+        assert np.array_equal(S, np.transpose(K, (1, 3, 2, 0)))
+    K = M.interaction_matrix(tris, mat, 0.1)
+    out = np.zeros((K.shape[1], K.shape[3], K.shape[2], K.shape[0]), np.float32)
+    consumer.interaction_source_major(tris, mat, 0.1, out=out)
+    assert np.array_equal(out, np.transpose(K, (1, 3, 2, 0)).astype(np.float32))
+    D = consumer.interaction_source_major(tris, mat, 0.1, device=True)
+    assert isinstance(D, torch.Tensor) and D.is_cuda
+    assert np.array_equal(D.cpu().numpy(), np.transpose(K, (1, 3, 2, 0)))
+
+
 def test_identities_on_gpu(M, small):
     """total - eigenstress == elastic to rounding; interaction == stress
     contracted with the receiver plane and both frames (1e-13,
