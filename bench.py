@@ -116,6 +116,37 @@ def counter_pass(mode: str) -> None:
     _progress(f"counter pass done ({mode}, default column tiles per block)")
 
 
+def source_major_timing() -> list:
+    """HIP-event time of the source-major copy (mhs_source_major) of
+    interaction-shaped device matrices: (receivers, sources, n_a, n_b) ->
+    (sources, n_b, n_a, receivers); bytes = 8 read + 8 written per entry."""
+    import ctypes as C
+    import torch
+    from mhs_amd import _lib
+    lib = _lib.load()
+    rows = []
+    for (nr, ns, na, nb) in ((10000, 10000, 1, 1), (2000, 2000, 3, 3), (12000, 12000, 3, 2)):
+        a = torch.randn(nr, ns, na, nb, dtype=torch.float64, device="cuda")
+        b = torch.empty(ns, nb, na, nr, dtype=torch.float64, device="cuda")
+        st = torch.cuda.current_stream()
+        h = C.c_void_p(st.cuda_stream)
+        args = (C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()), nr, ns, na, nb, h)
+        _lib.check(lib.mhs_source_major(*args), lib)
+        torch.cuda.synchronize()
+        if not torch.equal(b, a.permute(1, 3, 2, 0)):
+            raise RuntimeError("source-major copy differs from the permutation")
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(10):
+            _lib.check(lib.mhs_source_major(*args), lib)
+        e1.record(st)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / 10
+        rows.append({"shape": [nr, ns, na, nb], "ms": ms, "GB/s": 16 * a.numel() / ms / 1e6})
+        del a, b
+    return rows
+
+
 def _progress(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -195,6 +226,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=512,
                     help="requests of the served stream (0 = skip it)")
+    ap.add_argument("--source-major", action="store_true",
+                    help="time the device source-major copy of interaction-shaped matrices and exit")
     ap.add_argument("--counter-pass", action="store_true",
                     help="run one far pass of --mode plus a fill and a copy of 256 MiB, print nothing "
                          "(the program profiled by the FETCH_SIZE / WRITE_SIZE counter runs)")
@@ -206,6 +239,9 @@ def main() -> int:
         return 2
     if args.counter_pass:
         counter_pass(args.mode)
+        return 0
+    if args.source_major:
+        print(json.dumps({"source_major": source_major_timing()}))
         return 0
 
     from mhs_amd import flops
