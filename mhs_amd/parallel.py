@@ -29,15 +29,20 @@ stitched matrix is bitwise the one a single call returns.
 ``by_source`` cuts columns, which are disjoint only in the discontinuous
 layout; the continuous layout (shared nodes) is cut by receiver rows with
 ``by_rows``, as in the reference.
+
+:func:`stream_rows` is the out-of-core path of this package's own calls:
+receiver-row blocks of one validated call, each block's device-to-host copy
+overlapping the next block's assembly, into a host array or memmap.
 """
 from __future__ import annotations
 
+import dataclasses
 import os
 from contextlib import contextmanager
 
 import numpy as np
 
-from . import defaults
+from . import api, defaults, runtime
 
 #: Threading variables of the BLAS libraries (kept for callers that use the
 #: reference's ``pinned_blas``; this package's device path uses no BLAS).
@@ -185,4 +190,77 @@ def by_rows(call, n_rows: int, *, workers: int | None = None, out=None,
     del first
     for a, b in _device_blocks(n_rows - probe_hi, bytes_per_row, block_bytes):
         buf[probe_hi + a:probe_hi + b] = run(probe_hi + a, probe_hi + b)
+    return buf
+
+
+def stream_rows(fn, *args, out=None, block_bytes: int | None = None, **kwargs):
+    """``fn(*args, **kwargs)`` -- any public matrix call of this package --
+    assembled by receiver-row blocks straight into ``out`` (a host ndarray
+    or ``numpy.memmap`` of the call's shape, float64 or float32; else a new
+    float64 array), with each block's device-to-host copy overlapping the
+    next block's assembly.
+
+    The row blocks are cut from the validated call (receivers are
+    independent in every layout), each block's result kept under
+    ``block_bytes`` of device memory (default
+    ``defaults.DEVICE_BLOCK_BYTES``), two pinned host staging buffers, the
+    assembly on one stream and the copies on another.  The matrix equals the
+    direct call's bitwise.  This is the out-of-core path: neither the device
+    nor host RAM ever holds more than two blocks (the output itself may be a
+    memmap).  ``parts=True`` returns two matrices and is not streamed."""
+    job = api.plan(fn, *args, **kwargs)
+    if job.parts:
+        raise ValueError("stream_rows streams one matrix; parts=True wants by_rows")
+    shape = job.out_shape()
+    if out is None:
+        buf = np.empty(shape, dtype=np.float64)
+    else:
+        if not isinstance(out, np.ndarray):
+            raise TypeError(f"out must be a numpy.ndarray, got {type(out).__name__}")
+        if out.shape != shape:
+            raise ValueError(f"out has shape {out.shape}, expected {shape}")
+        if out.dtype not in (np.float64, np.float32):
+            raise ValueError(f"out must be float64 or float32, got {out.dtype}")
+        buf = out
+    n = job.n_obs
+    row_elems = int(np.prod(shape[1:]))
+    blocks = _device_blocks(n, 8 * row_elems, block_bytes)
+    if not blocks:
+        return buf
+    dev = runtime.device()
+    torch = dev.torch
+    compute, copy = torch.cuda.Stream(device=dev.dev), torch.cuda.Stream(device=dev.dev)
+    most = max(hi - lo for lo, hi in blocks) * row_elems
+    staging = [torch.empty(most, dtype=torch.float64, pin_memory=True) for _ in range(2)]
+
+    def rows_of(lo, hi):
+        cut = {"obs": job.obs[lo:hi]}
+        for name in ("nrm", "rb"):
+            if getattr(job, name) is not None:
+                cut[name] = getattr(job, name)[lo:hi]
+        return dataclasses.replace(job, **cut)
+
+    def land(item):
+        lo, hi, host, done, prep = item
+        done.synchronize()
+        prep.check_fold()
+        buf[lo:hi] = host[: (hi - lo) * row_elems].numpy().reshape((hi - lo,) + shape[1:])
+
+    pending = None
+    for i, (lo, hi) in enumerate(blocks):
+        prep = runtime.Prepared(rows_of(lo, hi), stream=compute, dev=dev)
+        with torch.cuda.stream(compute):
+            prep.launch()
+            assembled = torch.cuda.Event()
+            assembled.record(compute)
+        copy.wait_event(assembled)
+        host = staging[i % 2]
+        with torch.cuda.stream(copy):
+            host[: (hi - lo) * row_elems].copy_(prep.final.reshape(-1), non_blocking=True)
+            done = torch.cuda.Event()
+            done.record(copy)
+        if pending is not None:
+            land(pending)           # the previous block, while this one assembles
+        pending = (lo, hi, host, done, prep)
+    land(pending)
     return buf

@@ -7,6 +7,7 @@ fallback: without a GPU or without the built library this module raises.
 from __future__ import annotations
 
 import ctypes as C
+import threading
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -435,12 +436,60 @@ def run(job: Job, stream=None, dev: Device | None = None):
     return prep.final, prep.final2
 
 
+#: Device results larger than this go to the host through pinned staging.
+STAGED_COPY_BYTES = 16 << 20
+#: Bytes per pinned staging buffer (two per process, reused).
+STAGE_BYTES = 64 << 20
+_STAGING: dict = {}
+_STAGING_LOCK = threading.Lock()
+
+
 def to_host(t, out=None):
-    """Copy a device tensor into ``out`` (float64 or float32 ndarray) or a new array."""
+    """Copy a device tensor into ``out`` (float64 or float32 ndarray) or a
+    new float64 array.  Large results go through two pinned staging buffers
+    on a copy stream: the device-to-host copy of one chunk overlaps the host
+    copy of the previous one into ``out`` (a pageable-memory copy of the
+    whole tensor runs several times slower)."""
+    n = int(t.numel())
     if out is None:
-        return t.cpu().numpy()
-    if out.dtype == np.float32:
-        out[...] = t.float().cpu().numpy()
-    else:
-        out[...] = t.cpu().numpy()
+        out = np.empty(tuple(t.shape), dtype=np.float64)
+    if n * 8 < STAGED_COPY_BYTES or not out.flags.c_contiguous:
+        if out.dtype == np.float32:
+            out[...] = t.float().cpu().numpy()
+        else:
+            out[...] = t.cpu().numpy()
+        return out
+    with _STAGING_LOCK:
+        return _staged_copy(t, out, n)
+
+
+def _staged_copy(t, out, n):
+    torch = _torch()
+    dev = t.device
+    key = (dev.index, STAGE_BYTES)
+    if key not in _STAGING:
+        ch = STAGE_BYTES // 8
+        _STAGING[key] = ([torch.empty(ch, dtype=torch.float64, pin_memory=True) for _ in range(2)],
+                         torch.cuda.Stream(device=dev))
+    bufs, copy = _STAGING[key]
+    ch = int(bufs[0].numel())
+    src = t.reshape(-1)
+    dst = out.reshape(-1)
+    copy.wait_stream(torch.cuda.current_stream(dev))
+    pending = None
+    for i, lo in enumerate(range(0, n, ch)):
+        hi = min(n, lo + ch)
+        b = bufs[i % 2]
+        with torch.cuda.stream(copy):
+            b[: hi - lo].copy_(src[lo:hi], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(copy)
+        if pending is not None:
+            plo, phi, pb, pev = pending
+            pev.synchronize()
+            dst[plo:phi] = pb[: phi - plo].numpy()
+        pending = (lo, hi, b, ev)
+    plo, phi, pb, pev = pending
+    pev.synchronize()
+    dst[plo:phi] = pb[: phi - plo].numpy()
     return out

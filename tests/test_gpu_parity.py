@@ -192,6 +192,41 @@ def test_block_drivers_equal_one_call(M):
     assert np.array_equal(mp.by_rows(call, full.shape[0], block_bytes=100_000), full)
 
 
+def test_stream_rows_equals_one_call(M, tmp_path):
+    """Row blocks with overlapped device-to-host copies land bitwise the
+    direct call's matrix: displacement, stress, traction, the contracted and
+    the continuous interaction; a memmap and a float32 out=."""
+    from mhs_amd import parallel as mp, workloads
+    rng = np.random.default_rng(5)
+    Pl = workloads.plane(10.0, 5.0, 8, 4, 50.0, 0.0)
+    tris = workloads.orient_up(workloads.grid_mesh(Pl + rng.uniform(-0.05, 0.05, Pl.shape) * [1, 1, 0]))
+    tris[:, :, 2] = np.minimum(tris[:, :, 2], 0.0)
+    mat = M.Material(30.0, 30.0)
+    obs = np.column_stack([rng.uniform(-8, 8, 70), rng.uniform(-8, 8, 70), -rng.uniform(0, 5, 70)])
+    nrm = rng.normal(size=(70, 3))
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    cases = [(M.disp_matrix, (obs, tris, mat, 0.1), {"order": 1}),
+             (M.stress_matrix, (obs, tris, mat, 0.1), {}),
+             (M.traction_matrix, (obs, nrm, tris, mat, 0.1), {"order": 2}),
+             (M.interaction_matrix, (tris, mat, 0.1), {"receiver": "strike", "source": "strike"}),
+             (M.interaction_matrix, (tris, mat, 0.1), {"order": 2, "layout": "continuous",
+                                                       "shared": M.shared_nodes(tris)})]
+    for fn, a, kw in cases:
+        print(f"case stream_rows {fn.__name__} {kw.get('order', 0)}", flush=True)
+        want = fn(*a, **kw)
+        assert np.array_equal(mp.stream_rows(fn, *a, block_bytes=want[0].nbytes * 7, **kw), want)
+    want = M.traction_matrix(obs, nrm, tris, mat, 0.1, order=2)
+    mm = np.lib.format.open_memmap(tmp_path / "t.npy", mode="w+", dtype=np.float64, shape=want.shape)
+    mp.stream_rows(M.traction_matrix, obs, nrm, tris, mat, 0.1, order=2, out=mm, block_bytes=want[0].nbytes * 9)
+    mm.flush()
+    assert np.array_equal(np.load(tmp_path / "t.npy"), want)
+    o32 = np.zeros(want.shape, np.float32)
+    mp.stream_rows(M.traction_matrix, obs, nrm, tris, mat, 0.1, order=2, out=o32)
+    assert np.array_equal(o32, want.astype(np.float32))
+    with pytest.raises(ValueError, match="parts=True"):
+        mp.stream_rows(M.stress_matrix, obs, tris, mat, 0.1, parts=True)
+
+
 def test_source_major_layout(M):
     """The consumer layout (n_src, n_b, n_a, n_rec) copied on the device is
     bitwise the transpose of interaction_matrix, for 1, 2, 6 and 9
