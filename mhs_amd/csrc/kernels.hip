@@ -2,18 +2,23 @@
 // C-ABI they are launched through (declared in include/mhs_amd.h).
 //
 // Launch plan of one matrix call (reference: src/mhs/kernels/assemble.py),
-// observer rows in chunks, a 2-D grid (row, column tile) per kernel:
+// observer rows in chunks, a 2-D grid (row, column or source tile) per kernel:
 //   interaction with one or two components (contracted):
-//     k_far   the far field (direct, image R, image Q) as per-pair contracted
-//             pointwise series, written;
-//     k_near  the closed-form near field and the folded mirror, added;
+//     k_far   the far field (direct, image R, image Q) as contracted
+//             pointwise series per (row, source) pair, all its node columns
+//             from one evaluation of each point, written;
+//     k_near  the closed-form near field and the folded mirror of the pairs
+//             a row tile lists as having one, added;
 //   every other form:
 //     k_pair  direct + image R-family (or the folded mirror) + eigenstress,
-//             written in the output's layout;
+//             written in the output's layout (displacement: row-tile lists,
+//             far pairs first);
 //     k_qm    the image Q-family, moments first, added;
 //   then k_fold_list / k_fold_prep / k_fold (near and far points): the
 //   Boussinesq surface integral of the folded pairs, added; k_gather the
 //   continuous layout's shared-column sums; k_strain the elastic strain.
+//   k_source_major copies a finished interaction matrix to the consumers'
+//   source-major layout (its own entry point).
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <string.h>

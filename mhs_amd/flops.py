@@ -48,7 +48,11 @@ the far field pointwise instead (csrc/kernels.hip):
 * k_near runs the closed forms of the near branches (the constants above).
 
 The near-field constants are averages over the edge regimes; the quadrature
-ones are exact per point.
+ones are exact per point.  The model is stated at order 0, where a pair is
+one column.  At orders 1 and 2 the contracted far pass evaluates each point
+once per (row, source) pair for all K node columns (plus K weighted
+accumulations per point), so its count is that of n_cols / K pairs; the
+histogram's per-column counts overstate it K-fold.
 """
 from __future__ import annotations
 
