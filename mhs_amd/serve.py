@@ -71,6 +71,13 @@ class _Active:
         self.inflight = collections.deque()
 
 
+#: Device memory kept free of the tensor cache while serving: the HIP runtime
+#: allocates each hardware queue's scratch (the closed-form kernels hold up
+#: to 12.6 KB per lane, about 1.6 GB per queue at full occupancy) from what
+#: the cache has not reserved, and fails the queue when that runs out.
+SCRATCH_HEADROOM_BYTES = 16 << 30
+
+
 class GpuBackend:
     """Device side of the loop: one stream (default priority) per slot, the
     HIP kernels through :class:`runtime.Prepared`, completion by events."""
@@ -79,9 +86,17 @@ class GpuBackend:
         self.dev = dev or runtime.device()
         self.torch = self.dev.torch
         self.streams = [self.torch.cuda.Stream(device=self.dev.dev) for _ in range(slots)]
+        self.total = int(self.torch.cuda.get_device_properties(self.dev.dev).total_memory)
 
     def default_budget(self) -> int:
-        return int(0.8 * self.torch.cuda.get_device_properties(self.dev.dev).total_memory)
+        return int(0.8 * self.total)
+
+    def trim(self) -> None:
+        """Return the tensor cache's unused blocks to the device when it has
+        reserved more than all but SCRATCH_HEADROOM_BYTES of it (requests of
+        many sizes fragment the cache until it holds nearly all of HBM)."""
+        if self.torch.cuda.memory_reserved(self.dev.dev) > self.total - SCRATCH_HEADROOM_BYTES:
+            self.torch.cuda.empty_cache()
 
     def prepare(self, job: runtime.Job, slot: int):
         return runtime.Prepared(job, stream=self.streams[slot], dev=self.dev)
@@ -188,6 +203,8 @@ class Server:
                 free_slots.append(a.slot)
                 in_use -= a.bytes
                 a.prep = None
+            if finished and hasattr(be, "trim"):
+                be.trim()
             if not issued and not finished:
                 # nothing to do until a chunk completes or the next arrival:
                 # poll at a fine grain so that neither is noticed late

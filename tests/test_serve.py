@@ -124,3 +124,19 @@ def test_c5_stream_shapes():
         if job.kind == "interaction":
             assert job.n_obs <= serve.INTERACTION_MAX_DOFS
         assert runtime.job_bytes(job) > 0
+
+
+def test_cache_is_trimmed_as_requests_finish():
+    """The loop gives the backend a chance to return cached device memory
+    (headroom for the runtime's scratch) whenever requests complete."""
+    class Trimming(FakeBackend):
+        trims = 0
+
+        def trim(self):
+            self.trims += 1
+
+    be = Trimming()
+    reqs = [_disp_request(64, 0.0) for _ in range(6)]
+    done = serve.Server(slots=2, backend=be, chunk_pairs=1 << 14).run(reqs, keep=False, realtime=False)
+    assert all(s.error is None for s in done)
+    assert 1 <= be.trims <= len(reqs)
