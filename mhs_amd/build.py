@@ -17,6 +17,11 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          # packed FP32 instructions are kept out of every HIP build
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         # fully unroll every `#pragma unroll` loop of known trip count: at the
+         # default size limit the image edge integrals' 66-step series stays a
+         # loop that loads each binomial coefficient and a dynamically indexed
+         # accumulator from memory and waits on every step
+         "-mllvm", "-pragma-unroll-threshold=65536",
          "-I", str(HERE.parent / "include")]
 
 
