@@ -34,21 +34,21 @@ def interaction_source_major(tris, material, eps, *, out=None, device: bool = Fa
     ``device=True`` returns the device tensor instead of a host array.
     Validation and errors are ``interaction_matrix``'s."""
     job = api.plan(api.interaction_matrix, tris, material, eps, **kwargs)
-    prep = runtime.Prepared(job)
-    torch = prep.dev.torch
-    with torch.cuda.stream(prep.stream):
-        prep.launch()
-    prep.check_fold()
-    K = prep.final
-    n_rec, n_src, n_a, n_b = (int(v) for v in K.shape)
+    n_rec, n_src, n_a, n_b = job.out_shape()
     shape = (n_src, n_b, n_a, n_rec)
-    if out is not None:
+    if out is not None:                  # refused before any device work
         if not isinstance(out, np.ndarray):
             raise TypeError(f"out must be a numpy.ndarray, got {type(out).__name__}")
         if out.shape != shape:
             raise ValueError(f"out has shape {out.shape}, expected {shape}")
         if out.dtype not in (np.float64, np.float32):
             raise ValueError(f"out must be float64 or float32, got {out.dtype}")
+    prep = runtime.Prepared(job)
+    torch = prep.dev.torch
+    with torch.cuda.stream(prep.stream):
+        prep.launch()
+    prep.check_fold()
+    K = prep.final
     dst = torch.empty(shape, dtype=torch.float64, device=K.device)
     lib = prep.dev.lib
     with torch.cuda.stream(prep.stream):

@@ -42,3 +42,32 @@ def test_cache_key_per_source_eps_and_shared_nodes():
     b = consumer.cache_key(grid, mat, 0.1, order=2, layout="continuous", shared=M.shared_nodes(grid.copy()))
     assert a == b
     assert a != consumer.cache_key(grid, mat, 0.1, order=2, layout="continuous")
+
+
+def test_layout_and_stream_arguments_refused_before_device_work():
+    """Bad out= (type, shape, dtype) and parts=True for streaming are
+    refused from the validated plan alone, before anything reaches the
+    device (this runs without a GPU)."""
+    import pytest
+    from mhs_amd import parallel
+    from mhs_amd import workloads
+    tris = workloads.orient_up(workloads.grid_mesh(workloads.plane(4.0, 2.0, 2, 1, 60.0, -1.0)))
+    mat = M.Material(30.0, 30.0)
+    kw = dict(receiver="strike", source=("strike", "dip"))
+    n = tris.shape[0]
+    with pytest.raises(ValueError, match=rf"out has shape \(1,\), expected \({n}, 2, 1, {n}\)"):
+        consumer.interaction_source_major(tris, mat, 0.1, out=np.zeros(1), **kw)
+    with pytest.raises(ValueError, match="float64 or float32"):
+        consumer.interaction_source_major(tris, mat, 0.1, out=np.zeros((n, 2, 1, n), np.int64), **kw)
+    with pytest.raises(TypeError, match="numpy.ndarray"):
+        consumer.interaction_source_major(tris, mat, 0.1, out=[0.0], **kw)
+    obs = np.array([[0.0, 1.0, -0.5], [1.0, 0.0, -1.0]])
+    with pytest.raises(ValueError, match="parts=True"):
+        parallel.stream_rows(M.stress_matrix, obs, tris, mat, 0.1, parts=True)
+    with pytest.raises(ValueError, match=r"out has shape \(2, 3\), expected"):
+        parallel.stream_rows(M.disp_matrix, obs, tris, mat, 0.1, out=np.zeros((2, 3)))
+    with pytest.raises(ValueError, match="float64 or float32"):
+        parallel.stream_rows(M.disp_matrix, obs, tris, mat, 0.1, out=np.zeros((2, 3, n, 3), np.float16))
+    # the reference's own validation comes first, with its messages
+    with pytest.raises(ValueError, match="z"):
+        parallel.stream_rows(M.disp_matrix, obs * [1, 1, -1], tris, mat, 0.1)
