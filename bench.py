@@ -213,6 +213,7 @@ def _time_mode(mixed, steps, warmup):
         prep.launch()                    # leave a whole result for the error check
         torch.cuda.synchronize()
     stage_ms["far_tiles"] = prep.far_tiles
+    stage_ms["tune_ms"] = {str(k): round(v, 3) for k, v in prep.dev.tune_times.get(prep.tune_key(), {}).items()}
     return prep, dt, stage_ms
 
 

@@ -16,8 +16,10 @@ from . import _lib, geometry
 
 ROW_SCRATCH_BYTES = 4 << 30      # continuous-layout element-major scratch per launch
 FOLD_ITEM_BYTES = 20200          # upper bound of mhs_fold_item_bytes (checked by the ABI tests)
-FAR_TILE_CANDIDATES = (4, 8, 16, 32)   # column tiles per block tried by the autotuner
-AUTOTUNE_PAIRS = 1 << 22               # pairs of the sample each candidate is timed on
+FAR_TILE_CANDIDATES = (8, 16, 32)      # column tiles per block of the far pass the tuner tries
+AUTOTUNE_TIE = 0.05                    # candidates this close to the fastest count as tied: the most tiles wins
+AUTOTUNE_PAIRS = 1 << 24               # pairs of the sample each candidate is timed on
+AUTOTUNE_REPEATS = 2                   # timed launches per candidate (the fastest counts)
 autotune = True                         # off: the library default (16) is used
 
 PRECISIONS = ("fp64", "mixed")
@@ -132,6 +134,7 @@ class Device:
         self.gq = torch.from_numpy(geometry.gauss_triangle_tables()).to(self.dev)
         self._coef = {}
         self.far_tiles = {}     # autotuner cache: (order, outputs, mixed, log2 columns) -> tiles
+        self.tune_times = {}    # the same key -> {tiles: ms of the sample's far pass}
 
     def coef(self, mu: float, lam: float):
         key = (float(mu), float(lam))
@@ -267,11 +270,13 @@ class Prepared:
 
     def tune(self):
         """Pick the far pass's column tiles per block: each candidate is timed
-        (HIP events, after one warm launch) on the far pass over a sample of
-        the job's rows; the fastest is cached per (order, outputs, precision,
-        size class) on the device.  Only the grouping of pairs into blocks
-        changes, never a value; the sample rows are rewritten by the real
-        launch.  Synchronises the stream once per new key."""
+        (HIP events, after one warm launch, the faster of AUTOTUNE_REPEATS
+        launches) on the far pass over about AUTOTUNE_PAIRS pairs in three
+        blocks of rows spread over the matrix; the fastest is cached per
+        (order, outputs, precision, size class) on the device, with the
+        timings.  Only the grouping of pairs into blocks changes, never a
+        value; the sample rows are rewritten by the real launch.
+        Synchronises the stream once per candidate of a new key."""
         if not self.contracted or self.far_tiles:
             return self.far_tiles
         key = self.tune_key()
@@ -281,20 +286,40 @@ class Prepared:
             return self.far_tiles
         if not autotune:
             return 0
-        rows = int(min(self.job.n_obs, max(1, AUTOTUNE_PAIRS // max(self.n_cols, 1))))
+        n = self.job.n_obs
+        rows = int(min(n, max(1, AUTOTUNE_PAIRS // max(self.n_cols, 1))))
+        # three blocks of rows spread over the matrix (receivers near its
+        # ends and its middle see different near/far mixes), or all rows
+        if rows >= n:
+            blocks = [(0, n)]
+        else:
+            per = max(1, rows // 3)
+            blocks = sorted({(max(0, min(n - per, int(f * n) - per // 2)),
+                              max(0, min(n - per, int(f * n) - per // 2)) + per) for f in (1 / 6, 1 / 2, 5 / 6)})
         times = {}
         with torch.cuda.stream(self.stream):
             for ct in FAR_TILE_CANDIDATES:
                 self.far_tiles = ct
-                self.launch_rows(0, rows, _lib.STAGE_Q)
-                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                e0.record(self.stream)
-                self.launch_rows(0, rows, _lib.STAGE_Q)
-                e1.record(self.stream)
-                e1.synchronize()
-                times[ct] = e0.elapsed_time(e1)
-        best = min(times, key=times.get)
+                for lo, hi in blocks:
+                    self.launch_rows(lo, hi, _lib.STAGE_Q)
+                best_ms = float("inf")
+                for _ in range(AUTOTUNE_REPEATS):
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record(self.stream)
+                    for lo, hi in blocks:
+                        self.launch_rows(lo, hi, _lib.STAGE_Q)
+                    e1.record(self.stream)
+                    e1.synchronize()
+                    best_ms = min(best_ms, e0.elapsed_time(e1))
+                times[ct] = best_ms
+        # the sample separates the candidates by a few percent at most, less
+        # than the full launch's spread, so near-ties go to the most tiles per
+        # block (4 tiles per block sampled as fast as 32 and ran the full
+        # 10^4 x 10^4 far pass 6-8 % slower)
+        fastest = min(times.values())
+        best = max(ct for ct, t in times.items() if t <= fastest * (1.0 + AUTOTUNE_TIE))
         dev.far_tiles[key] = best
+        dev.tune_times[key] = times
         self.far_tiles = best
         return best
 
