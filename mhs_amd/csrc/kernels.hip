@@ -319,15 +319,33 @@ __device__ __forceinline__ void d_blocked(const T* __restrict__ kd, const T* x, 
 }
 
 // 1/sqrt and 1/x of the point series: the hardware approximations (1 ulp)
-// in FP32, correctly rounded in FP64.
+// in FP32; in FP64 the hardware approximations refined by two Newton steps
+// each (the seed's ~2^-22 relative error squares per step, to the last bits
+// of a double), about half the instructions of a correctly rounded sqrt and
+// two divisions.  R2 >= eps^2 > 0 and R - d3 >= R > 0 (d3 <= 0 on the
+// mirrored side), so no special values arise.
 __device__ __forceinline__ void radii(float R2, float d3, bool q, float& ir, float& iq) {
   ir = __builtin_amdgcn_rsqf(R2);
   iq = q ? __builtin_amdgcn_rcpf(R2 * ir - d3) : 0.0f;
 }
+__device__ __forceinline__ double rsq_refined(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const double r = fma(-x * y, y, 1.0);          // 1 - x y^2
+    y = fma(0.5 * y, r, y);
+  }
+  return y;
+}
+__device__ __forceinline__ double rcp_refined(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+#pragma unroll
+  for (int it = 0; it < 2; ++it) y = fma(y, fma(-x, y, 1.0), y);
+  return y;
+}
 __device__ __forceinline__ void radii(double R2, double d3, bool q, double& ir, double& iq) {
-  const double R = sqrt(R2);
-  ir = 1.0 / R;
-  iq = q ? 1.0 / (R - d3) : 0.0;
+  ir = rsq_refined(R2);
+  iq = q ? rcp_refined(R2 * ir - d3) : 0.0;
 }
 
 // One quadrature pass of a pair over the ng x ng collapsed Gauss points of
