@@ -69,6 +69,26 @@ MHS_INLINE void load_frame(const double* __restrict__ s, bool image, Frame& f) {
   f.L = image ? s[S_IL] : s[S_L];
 }
 
+// 1/sqrt(x) and 1/x in FP64 for the quadrature loops (x > 0, finite): the
+// hardware approximations (~2^-22 relative) refined by two Newton steps,
+// each squaring the error, to the last bits of a double -- about half the
+// instructions of a correctly rounded sqrt and division.
+__device__ __forceinline__ double rsq_refined(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const double r = fma(-x * y, y, 1.0);          // 1 - x y^2
+    y = fma(0.5 * y, r, y);
+  }
+  return y;
+}
+__device__ __forceinline__ double rcp_refined(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+#pragma unroll
+  for (int it = 0; it < 2; ++it) y = fma(y, fma(-x, y, 1.0), y);
+  return y;
+}
+
 // Triangular index of xi1^u xi2^v, and table sizes.
 __host__ __device__ constexpr int tix(int u, int v) { return (u + v) * (u + v + 1) / 2 + v; }
 __host__ __device__ constexpr int tsize(int d) { return d < 0 ? 0 : (d + 1) * (d + 2) / 2; }
@@ -588,7 +608,7 @@ MHS_INLINE void weighted_far(const Frame& f, int k, double X1, double X2, double
       const double wn = w * shape_at<P>(k, l0, x1, x2);
       const double xi1 = eta1 - X1, xi2 = eta2 - X2;
       const double R2 = xi1 * xi1 + xi2 * xi2 + h2;
-      const double ir = 1.0 / sqrt(R2);
+      const double ir = rsq_refined(R2);
       const double ir2 = ir * ir;
       double rn[6];
       rn[0] = R2 * ir;  // R^1 (unused)

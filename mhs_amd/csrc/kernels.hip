@@ -228,8 +228,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         const double y2 = l0 * v[2] + x1 * v[5] + x2 * v[8];
         const double wn = w * shape_at<P>(k, l0, x1, x2);
         const double d1 = ob[0] - y0, d2 = ob[1] - y1, d3 = ob[2] + y2;
-        const double R = sqrt(d1 * d1 + d2 * d2 + d3 * d3 + eps2);
-        const double ir = 1.0 / R, iq = 1.0 / (R - d3);
+        const double R2 = d1 * d1 + d2 * d2 + d3 * d3 + eps2;
+        const double ir = rsq_refined(R2), iq = rcp_refined(R2 * ir - d3);
         if (STRESS) qmom_ddg_acc(c, d1, d2, d3, ir, iq, wn, M);
         else qmom_dg_acc(c, d1, d2, d3, ir, iq, wn, M);
       }
@@ -327,21 +327,6 @@ __device__ __forceinline__ void d_blocked(const T* __restrict__ kd, const T* x, 
 __device__ __forceinline__ void radii(float R2, float d3, bool q, float& ir, float& iq) {
   ir = __builtin_amdgcn_rsqf(R2);
   iq = q ? __builtin_amdgcn_rcpf(R2 * ir - d3) : 0.0f;
-}
-__device__ __forceinline__ double rsq_refined(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-#pragma unroll
-  for (int it = 0; it < 2; ++it) {
-    const double r = fma(-x * y, y, 1.0);          // 1 - x y^2
-    y = fma(0.5 * y, r, y);
-  }
-  return y;
-}
-__device__ __forceinline__ double rcp_refined(double x) {
-  double y = __builtin_amdgcn_rcp(x);
-#pragma unroll
-  for (int it = 0; it < 2; ++it) y = fma(y, fma(-x, y, 1.0), y);
-  return y;
 }
 __device__ __forceinline__ void radii(double R2, double d3, bool q, double& ir, double& iq) {
   ir = rsq_refined(R2);
