@@ -623,36 +623,33 @@ k_far(Params pr) {
 // One pair's near parts as a real call: inlined into the list loop its
 // closed forms spill far more than as a function of its own.
 template <int P>
-__device__ __attribute__((noinline)) void near_src_call(const Params& pr, int64_t o, int64_t item) {
-  // order 2: one (row, source) pair, all six node columns from one set of
-  // closed-form raw moments; orders 0 and 1: one column (the all-nodes form
-  // holds more scratch, which at three nodes or fewer costs more than the
-  // shared moments save: measured at order 1, 24 ms against 17 ms)
-  if (P == 2) pair_near_src<P>(pr, o, item);
-  else pair_near_item<P>(pr, o, item);
+__device__ __attribute__((noinline)) void near_item_call(const Params& pr, int64_t o, int64_t col) {
+  pair_near_item<P>(pr, o, col);
 }
 
-// A block is one receiver row x MHS_ROW_TILE items (sources at order 2,
-// node columns otherwise); about one pair in nine has a near part, so the
-// block lists its items that do (those with an image or mirror part first,
-// then the direct-only ones) and evaluates the list with every lane busy.
+// A block is one receiver row x MHS_ROW_TILE node columns; about one pair in
+// nine has a near part, so the block lists its columns that do (those with
+// an image or mirror part first, then the direct-only ones) and evaluates the
+// list with every lane busy, a column per thread.  (Evaluating all K node
+// columns of a source from one set of closed-form raw moments held in scratch
+// was faster at order 2 while the image edge integrals ran as a memory-bound
+// loop, and slower once they were unrolled: 84 against 66 ms at n = 1560.)
 template <int P>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MHS_NEAR_WAVES))) k_near(Params pr) {
   constexpr int K = (P + 1) * (P + 2) / 2;
-  constexpr int PER = P == 2 ? 1 : K;          // node columns per source of an item index
   __shared__ int list_s[2 * MHS_ROW_TILE];
   const int64_t o = pr.obs_lo + blockIdx.y;
   const int64_t c0 = (int64_t)blockIdx.x * MHS_ROW_TILE;
   const double ob[3] = {pr.obs[3 * o], pr.obs[3 * o + 1], pr.obs[3 * o + 2]};
   int n_img, n_dir;
-  list_row_tile(P == 2 ? pr.n_src : pr.n_cols, c0, [&](int64_t i) {
-    const int parts = near_parts(pr, pr.src + (i / PER) * MHS_SRC_STRIDE, ob);
+  list_row_tile(pr.n_cols, c0, [&](int64_t col) {
+    const int parts = near_parts(pr, pr.src + (col / K) * MHS_SRC_STRIDE, ob);
     return (parts & 2) ? 1 : parts == 1 ? 2 : 0;
   }, list_s, n_img, n_dir);
   for (int i = threadIdx.x; i < n_img + n_dir; i += 256) {
     __asm__ volatile("" ::: "memory");   // no loads hoisted out of the pair loop
     const int c = i < n_img ? list_s[i] : list_s[MHS_ROW_TILE + i - n_img];
-    near_src_call<P>(pr, o, c0 + c);
+    near_item_call<P>(pr, o, c0 + c);
   }
 }
 
@@ -1037,8 +1034,7 @@ static int launch_all(const mhs_launch* L, const Params& pr, hipStream_t st) {
         if (check(hipGetLastError(), "k_far launch")) return -1;
       }
       if (stages & MHS_STAGE_PAIR) {
-        const dim3 src_tiles((unsigned)((pr.n_src + MHS_ROW_TILE - 1) / MHS_ROW_TILE), (unsigned)rows);
-        hipLaunchKernelGGL((k_near<P>), P == 2 ? src_tiles : tiles, block, 0, st, pc);
+        hipLaunchKernelGGL((k_near<P>), tiles, block, 0, st, pc);
         if (check(hipGetLastError(), "k_near launch")) return -1;
       }
       continue;

@@ -327,7 +327,7 @@ MHS_INLINE void pair_item(const Params& pr, int64_t o, int64_t col) {
 }
 
 // Which near parts a pair of the contracted interaction has, by the same
-// tests as pair_near_item and pair_near_src: bit 0 the closed-form direct term, bit 1 the
+// tests as pair_near_item: bit 0 the closed-form direct term, bit 1 the
 // folded mirror (band) or the closed-form image R-family.
 MHS_INLINE int near_parts(const Params& pr, const double* __restrict__ src, const double ob[3]) {
   const double eps = src[S_EPS];
@@ -427,119 +427,5 @@ MHS_INLINE void pair_near_item(const Params& pr, int64_t o, int64_t col) {
   }
   const int64_t oo = o - pr.out_lo;
   for (int t = 0; t < n_a * n_b; ++t) pr.out[(oo * pr.n_cols + col) * (n_a * n_b) + t] += acc[t];
-}
-
-// The near parts of the contracted interaction for every node column of
-// (receiver o, source s), orders 1 and 2: the same terms as pair_near_item,
-// but the closed-form raw moments depend on the pair only, so they are
-// computed once per part and weighted per node.
-template <int P>
-MHS_INLINE void pair_near_src(const Params& pr, int64_t o, int64_t s) {
-  constexpr int K = (P + 1) * (P + 2) / 2;
-  const double* src = pr.src + s * MHS_SRC_STRIDE;
-  const double ob[3] = {pr.obs[3 * o], pr.obs[3 * o + 1], pr.obs[3 * o + 2]};
-  const double eps = src[S_EPS];
-  const bool band = in_fold_band(src, ob, pr.flags);
-  const bool dnear = centroid_distance(src + S_C, ob, eps) <= 10.0 * src[S_L];
-  const bool rnear = !band && centroid_distance(src + S_IC, ob, eps) <= 10.0 * src[S_IL];
-  if (!dnear && !band && !rnear) return;
-  const Mat mt{pr.mu, pr.lam, pr.nu};
-  const bool sub = (pr.flags & F_SUBTRACT) != 0;
-  const int n_a = pr.n_a, n_b = pr.n_b;     // n_a * n_b <= 2
-  const double* nr = pr.nrm + 3 * o;
-  // receiver halves E_a, source halves F_b (element) and G_b (folded mirror:
-  // slip (-sx, -sy, sz) on the mirror normal, fold.py:370-380)
-  double Ea[2][6], Fb[2][6], Gb[2][6], rF[2][2], rG[2][2];
-  for (int a = 0; a < n_a; ++a) half_tensor(pr.rb + ((int64_t)o * n_a + a) * 3, nr, pr.mu, pr.lam, Ea[a]);
-  for (int b = 0; b < n_b; ++b) {
-    const double* sb = pr.sb + (s * n_b + b) * 3;
-    half_tensor(sb, src + S_N, pr.mu, pr.lam, Fb[b]);
-    const double sm[3] = {-sb[0], -sb[1], sb[2]};
-    half_tensor(sm, src + S_IN, pr.mu, pr.lam, Gb[b]);
-    for (int a = 0; a < n_a; ++a) {
-      const double* rb = pr.rb + ((int64_t)o * n_a + a) * 3;
-      double f = 0.0, g = 0.0;
-      const int ix[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
-      for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) {
-          f += rb[i] * Fb[b][ix[i][j]] * nr[j];
-          g += rb[i] * Gb[b][ix[i][j]] * nr[j];
-        }
-      rF[a][b] = f;
-      rG[a][b] = g;
-    }
-  }
-  double acc[K][2];
-  for (int k = 0; k < K; ++k) acc[k][0] = acc[k][1] = 0.0;
-  if (dnear) {
-    double M[6][36];
-    double W[6][21];
-    double c[6];
-    Frame f;
-    load_frame(src, false, f);
-    double X1, X2, h2;
-    const double z = frame_local(f, ob, eps, X1, X2, h2);
-    raw_moments_near<SpecH<P>>(f, src + S_V, X1, X2, h2, M);
-    for (int k = 0; k < K; ++k) {
-      shape_coeffs<P>(f, k, X1, X2, c);
-      weight_from_raw<SpecH<P>, P>(M, c, W);
-      for (int a = 0; a < n_a; ++a)
-        for (int b = 0; b < n_b; ++b)
-          acc[k][a * n_b + b] += direct_contract(f, W, z, eps, mt, Ea[a], Fb[b], sub, rF[a][b]);
-    }
-  }
-  if (band || rnear) {
-    Frame g;
-    load_frame(src, true, g);
-    double vi[9];
-    reflected_vertices(src, vi);
-    if (band) {
-      // the mirror element, closed form or Gauss by its own distance (rare:
-      // the fold band), one node at a time
-      double W[6][21];
-      double X1, X2;
-      for (int k = 0; k < K; ++k) {
-        const double z = weighted_tables<SpecH<P>, P>(g, vi, k, ob, eps, pr.gl_x, pr.gl_w, W, X1, X2);
-        for (int a = 0; a < n_a; ++a)
-          for (int b = 0; b < n_b; ++b)
-            acc[k][a * n_b + b] += direct_contract(g, W, z, eps, mt, Ea[a], Gb[b], sub, rG[a][b]);
-      }
-    } else {
-      double M[6][36];
-      double W[6][21];
-      double c[6];
-      double X1, X2, h2;
-      const double zp = frame_local(g, ob, eps, X1, X2, h2);
-      raw_moments_near<SpecI<P>>(g, vi, X1, X2, h2, M);
-      const double al[3] = {-g.e1[0], -g.e1[1], -g.e1[2]};
-      const double be[3] = {-g.e2[0], -g.e2[1], -g.e2[2]};
-      const double ga[3] = {zp * g.n[0], zp * g.n[1], zp * g.n[2]};
-      for (int k = 0; k < K; ++k) {
-        shape_coeffs<P>(g, k, X1, X2, c);
-        weight_from_raw<SpecI<P>, P>(M, c, W);
-        double DDG[81];
-#pragma unroll
-        for (int t = 0; t < 81; ++t) DDG[t] = 0.0;
-        r_ddg(W[2], W[3], W[4], W[5], al, be, ga, ob[2], pr.coef, DDG);
-        // rb.H.nr.sb of the R-family = sum over slots E[i][p] F[j][m] DDG[i][j][p][m]
-        const int ix[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
-        for (int a = 0; a < n_a; ++a)
-          for (int b = 0; b < n_b; ++b) {
-            double v = 0.0;
-#pragma unroll
-            for (int t = 0; t < 81; ++t) {
-              const int i = t / 27, j = (t / 9) % 3, p = (t / 3) % 3, m = t % 3;
-              v = fma(Ea[a][ix[i][p]] * Fb[b][ix[j][m]], DDG[t], v);
-            }
-            acc[k][a * n_b + b] += v;
-          }
-      }
-    }
-  }
-  const int64_t oo = o - pr.out_lo;
-  const int nout = n_a * n_b;
-  double* dst = pr.out + (oo * pr.n_cols + s * K) * nout;
-  for (int k = 0; k < K; ++k)
-    for (int t = 0; t < nout; ++t) dst[k * nout + t] += acc[k][t];
 }
 
