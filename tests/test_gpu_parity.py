@@ -192,6 +192,41 @@ def test_block_drivers_equal_one_call(M):
     assert np.array_equal(mp.by_rows(call, full.shape[0], block_bytes=100_000), full)
 
 
+def test_row_chunks_beyond_one_launch(M):
+    """More receiver rows than one launch takes (32,768): the launch loop's
+    row offsets, for the pair, Q-family, far, near and fold kernels.  Rows on
+    both sides of the chunk boundary against the FP64 oracle; the whole
+    matrix bitwise against row-streamed blocks (other launch geometry)."""
+    from mhs_amd import parallel as mp, workloads
+    from oracle import api as O
+    rng = np.random.default_rng(21)
+    Pl = workloads.plane(2.0, 1.0, 2, 1, 60.0, 0.0)
+    tris = workloads.orient_up(workloads.grid_mesh(Pl + rng.uniform(-0.02, 0.02, Pl.shape) * [1, 1, 0]))
+    tris[:, :, 2] = np.minimum(tris[:, :, 2], 0.0)
+    mat = M.Material(30.0, 30.0)
+    n = 40_000
+    obs = np.column_stack([rng.uniform(-3, 3, n), rng.uniform(-3, 3, n), -rng.uniform(0, 1.5, n)])
+    obs[32_760:32_780, 2] = -rng.uniform(0, 0.3, 20)            # fold band around the boundary
+    pick = np.r_[0, 32_766:32_770, n - 1]
+    print("case row chunks traction", flush=True)
+    nrm = rng.normal(size=(n, 3))
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    T = M.traction_matrix(obs, nrm, tris, mat, 0.1, order=1)
+    assert relmax(T[pick], O.traction_matrix(obs[pick], nrm[pick], tris, mat.mu, mat.lam, 0.1, order=1)) < TOL
+    assert np.array_equal(mp.stream_rows(M.traction_matrix, obs, nrm, tris, mat, 0.1, order=1,
+                                         block_bytes=T[0].nbytes * 9000), T)
+    print("case row chunks contracted interaction", flush=True)
+    otris = np.repeat(tris[:1], n, axis=0)
+    kw = dict(receiver="strike", source="strike", obs_tris=otris, obs_pts=obs)
+    K = M.interaction_matrix(tris, mat, 0.1, **kw)
+    ref = O.interaction_matrix(tris, mat.mu, mat.lam, 0.1, receiver="strike", source="strike",
+                               obs_tris=otris[pick], obs_pts=obs[pick])
+    assert relmax(K[pick], ref) < TOL
+    with M.precision("mixed"):
+        Km = M.interaction_matrix(tris, mat, 0.1, **kw)
+    assert relmax(Km, K) < 1e-6
+
+
 def test_stream_rows_equals_one_call(M, tmp_path):
     """Row blocks with overlapped device-to-host copies land bitwise the
     direct call's matrix: displacement, stress, traction, the contracted and
