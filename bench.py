@@ -147,6 +147,37 @@ def source_major_timing() -> list:
     return rows
 
 
+def shared_nodes_timing() -> list:
+    """Wall time of the continuous layout's shared-node table, host
+    (numpy, mhs_amd.shared.shared_nodes) against device
+    (shared_nodes_device, host-to-device and back included), on the
+    six-thousand-triangle three-fault network and on single dipping planes
+    of 40,000 and 490,000 triangles; the tables are checked equal."""
+    from mhs_amd import workloads
+    from mhs_amd.shared import shared_nodes, shared_nodes_device
+    cases = [("three-fault network, grouped",) + workloads.c4_network()[:2]]
+    for n in (200, 700):
+        P = workloads.plane(100.0, 50.0, n, n // 2, 60.0, -1.0)
+        cases.append((f"plane {n} x {n // 2} cells", workloads.orient_up(workloads.grid_mesh(P)), None))
+    rows = []
+    for name, tris, groups in cases:
+        t0 = time.perf_counter()
+        host = shared_nodes(tris, groups=groups)
+        t_host = time.perf_counter() - t0
+        shared_nodes_device(tris, groups=groups)              # warm: library, allocator
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            dev = shared_nodes_device(tris, groups=groups)
+            times.append(time.perf_counter() - t0)
+        if not (np.array_equal(dev.ids, host.ids) and np.array_equal(dev.boundary, host.boundary)):
+            raise RuntimeError(f"device shared-node table differs from the host's on {name}")
+        rows.append({"mesh": name, "n_tri": int(tris.shape[0]), "nodes": int(host.n),
+                     "host_s": t_host, "device_s": float(np.median(times)),
+                     "speedup": t_host / float(np.median(times))})
+    return rows
+
+
 def _progress(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -229,6 +260,8 @@ def main() -> int:
                     help="requests of the served stream (0 = skip it)")
     ap.add_argument("--source-major", action="store_true",
                     help="time the device source-major copy of interaction-shaped matrices and exit")
+    ap.add_argument("--shared-nodes", action="store_true",
+                    help="time the shared-node table, host against device, and exit")
     ap.add_argument("--counter-pass", action="store_true",
                     help="run one far pass of --mode plus a fill and a copy of 256 MiB, print nothing "
                          "(the program profiled by the FETCH_SIZE / WRITE_SIZE counter runs)")
@@ -240,6 +273,9 @@ def main() -> int:
         return 2
     if args.counter_pass:
         counter_pass(args.mode)
+        return 0
+    if args.shared_nodes:
+        print(json.dumps({"shared_nodes": shared_nodes_timing()}))
         return 0
     if args.source_major:
         print(json.dumps({"source_major": source_major_timing()}))

@@ -118,6 +118,29 @@ int mhs_strain(double* buf, int64_t n_obs, int64_t n_cols, double mu, double lam
 int mhs_source_major(const double* in, double* out, int64_t n_rec, int64_t n_src, int n_a, int n_b,
                      void* stream);
 
+/* Workspace bytes of mhs_shared_nodes for n_tri triangles (-1 if out of
+ * range: 1 <= 3 n_tri < 2^31). */
+int64_t mhs_weld_workspace_bytes(int64_t n_tri);
+
+/* Shared-node table of the continuous layout (shared.py:175-273,
+ * shared_nodes) on the device, for large fault networks.  Inputs: tris
+ * (n_tri, 3, 3), groups (n_tri) int64, tol = tol_over_h * shortest edge,
+ * area_normals (n_tri, 3) = area * upward unit normal and area3 (n_tri) =
+ * area / 3, all computed by the caller as the host table computes them.
+ * Outputs at capacity, n_v + n_e <= 6 n_tri nodes: ids (n_tri, 6) (v1 v2 v3
+ * m12 m23 m31 node ids), coords / normals (nodes, 3), weights (nodes),
+ * group (nodes) int64, boundary (nodes) uint8, edges (3 n_tri, 2) welded
+ * vertex pairs, counts and n_forward (3 n_tri) int32 (triangles on the edge,
+ * forward traversals), n_nodes[2] = {n_v, n_e} in host memory.  Node ids and
+ * the order of every sum equal the host table's.  Refusals (degenerate,
+ * non-manifold, inconsistent winding, soup) are the caller's, from ids,
+ * edges, counts and n_forward.  Synchronises the stream once. */
+int mhs_shared_nodes(const double* tris, const int64_t* groups, int64_t n_tri, double tol,
+                     const double* area_normals, const double* area3, void* work, int64_t work_bytes,
+                     int64_t* ids, double* coords, double* normals, double* weights, int64_t* group,
+                     uint8_t* boundary, int64_t* edges, int* counts, int* n_forward, int64_t* n_nodes,
+                     void* stream);
+
 /* Per-material image-kernel coefficients into host memory (mhs_ncoef()
  * doubles): the exact rational image table with nu = lam / (2 (lam + mu)) and
  * the scale 2 (lam + mu) / (pi mu (lam + 2 mu)) folded in

@@ -15,12 +15,12 @@ from .api import (collocation_points, disp_matrix, eigenstress_matrix, elastic_s
                   interaction_matrix, stress_matrix, total_stress_matrix, traction_matrix)
 from .materials import Material
 from .runtime import get_precision, precision, set_precision
-from .shared import SharedNodes, shared_nodes
+from .shared import SharedNodes, shared_nodes, shared_nodes_device
 
 __all__ = [
     "Material", "disp_matrix", "stress_matrix", "traction_matrix", "interaction_matrix",
     "total_stress_matrix", "eigenstress_matrix", "elastic_strain_matrix", "collocation_points",
-    "shared_nodes", "SharedNodes", "chunking", "consumer", "defaults", "parallel", "tdcs", "precision",
+    "shared_nodes", "shared_nodes_device", "SharedNodes", "chunking", "consumer", "defaults", "parallel", "tdcs", "precision",
     "set_precision", "get_precision",
 ]
 

@@ -39,7 +39,8 @@ class Launch(C.Structure):
 _LIB = None
 
 EXPORTS = ("mhs_assemble", "mhs_histogram", "mhs_gather", "mhs_strain", "mhs_fill_material", "mhs_ncoef",
-           "mhs_src_stride", "mhs_abi_version", "mhs_last_error", "mhs_fold_item_bytes", "mhs_source_major")
+           "mhs_src_stride", "mhs_abi_version", "mhs_last_error", "mhs_fold_item_bytes", "mhs_source_major",
+           "mhs_weld_workspace_bytes", "mhs_shared_nodes")
 
 
 def load(path=None):
@@ -66,6 +67,12 @@ def load(path=None):
     lib.mhs_fill_material.restype = C.c_int
     lib.mhs_source_major.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_int, C.c_int, C.c_void_p]
     lib.mhs_source_major.restype = C.c_int
+    lib.mhs_weld_workspace_bytes.argtypes = [C.c_int64]
+    lib.mhs_weld_workspace_bytes.restype = C.c_int64
+    lib.mhs_shared_nodes.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_double, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_int64] + [C.c_void_p] * 9 + [C.POINTER(C.c_int64),
+                                                                                  C.c_void_p]
+    lib.mhs_shared_nodes.restype = C.c_int
     lib.mhs_fold_item_bytes.argtypes = [C.c_int]
     lib.mhs_fold_item_bytes.restype = C.c_int64
     for nm in ("mhs_ncoef", "mhs_src_stride", "mhs_abi_version"):

@@ -11,6 +11,7 @@ import sys
 
 HERE = pathlib.Path(__file__).resolve().parent
 SRC = HERE / "csrc" / "kernels.hip"
+SRC_WELD = HERE / "csrc" / "weld.hip"
 OUT = HERE / "libmhs_amd.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -33,7 +34,7 @@ def build(verbose: bool = True) -> pathlib.Path:
     if not inc.exists() or inc.stat().st_mtime < table.stat().st_mtime:
         emit.main()
     tmp = OUT.with_suffix(".so.tmp")
-    cmd = [HIPCC, *FLAGS, "-o", str(tmp), str(SRC)]
+    cmd = [HIPCC, *FLAGS, "-o", str(tmp), str(SRC), str(SRC_WELD)]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -90,6 +90,12 @@ static int check(hipError_t e, const char* what) {
   return 0;
 }
 
+// The same message slot for the library's other translation unit (weld.hip).
+__attribute__((visibility("hidden"))) int mhs_internal_fail(const char* msg) { return fail(msg); }
+__attribute__((visibility("hidden"))) int mhs_internal_check(hipError_t e, const char* what) {
+  return check(e, what);
+}
+
 // Blocks of 256 for a grid-stride loop over n items, capped so the dispatch's
 // work-item count stays far inside 32 bits.
 static unsigned grid_1d(int64_t n) {

@@ -77,8 +77,8 @@ def _edges(idx):
     return edges, inv.reshape(-1, 3), counts
 
 
-def shared_nodes(tris, *, groups=None, tol_over_h=WELD_TOL_OVER_H) -> SharedNodes:
-    """Build the table (shared.py:175-273)."""
+def _prepare(tris, groups, tol_over_h):
+    """Validated float64 triangles, int64 groups and the weld tolerance."""
     tri = np.ascontiguousarray(np.asarray(tris, dtype=np.float64))
     if tri.ndim != 3 or tri.shape[1:] != (3, 3) or tri.shape[0] == 0:
         raise ValueError(f"tris must have shape (n_tri, 3, 3), got {tri.shape}")
@@ -92,7 +92,40 @@ def shared_nodes(tris, *, groups=None, tol_over_h=WELD_TOL_OVER_H) -> SharedNode
     h_min = float(np.linalg.norm(tri - np.roll(tri, 1, axis=1), axis=2).min())
     if not h_min > 0.0:
         raise ValueError("a triangle has a zero-length edge")
-    tol = float(tol_over_h) * h_min
+    return tri, np.ascontiguousarray(grp), float(tol_over_h) * h_min
+
+
+def _refuse_degenerate(idx, tol):
+    if (np.any(idx[:, 0] == idx[:, 1]) or np.any(idx[:, 1] == idx[:, 2])
+            or np.any(idx[:, 2] == idx[:, 0])):
+        raise ValueError(f"a triangle has two vertices within the weld tolerance "
+                         f"{tol:.3e}: degenerate, or tol_over_h too loose")
+
+
+def _refuse_edges(edges, counts, n_forward, n_tri):
+    """Non-manifold edges, a soup, inconsistent winding (shared.py:213-237)."""
+    if np.any(counts > 2):
+        bad = int(np.argmax(counts))
+        raise ValueError(f"non-manifold: edge {edges[bad].tolist()} is shared by "
+                         f"{int(counts[bad])} triangles")
+    if n_tri > 1 and not np.any(counts == 2):
+        raise ValueError(f"nothing is shared: {n_tri} triangles and no common edge; "
+                         f"a soup wants the discontinuous layout")
+    bad_wind = (counts == 2) & (n_forward != 1)
+    if np.any(bad_wind):
+        bad = int(np.argmax(bad_wind))
+        raise ValueError(f"inconsistent winding: the two triangles on edge "
+                         f"{edges[bad].tolist()} traverse it the same way")
+
+
+def _area(tri):
+    return 0.5 * np.linalg.norm(np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0]), axis=1)
+
+
+def shared_nodes(tris, *, groups=None, tol_over_h=WELD_TOL_OVER_H) -> SharedNodes:
+    """Build the table (shared.py:175-273)."""
+    tri, grp, tol = _prepare(tris, groups, tol_over_h)
+    n_tri = tri.shape[0]
     flat = tri.reshape(-1, 3)
     keys = np.concatenate([np.round(flat / tol).astype(np.int64),
                            np.repeat(grp, 3)[:, None]], axis=1)
@@ -103,26 +136,12 @@ def shared_nodes(tris, *, groups=None, tol_over_h=WELD_TOL_OVER_H) -> SharedNode
     np.add.at(verts, inv, flat)
     verts /= np.bincount(inv, minlength=n_v)[:, None]
     idx = inv.reshape(-1, 3)
-    if (np.any(idx[:, 0] == idx[:, 1]) or np.any(idx[:, 1] == idx[:, 2])
-            or np.any(idx[:, 2] == idx[:, 0])):
-        raise ValueError(f"a triangle has two vertices within the weld tolerance "
-                         f"{tol:.3e}: degenerate, or tol_over_h too loose")
+    _refuse_degenerate(idx, tol)
     edges, edge_of, counts = _edges(idx)
-    if np.any(counts > 2):
-        bad = int(np.argmax(counts))
-        raise ValueError(f"non-manifold: edge {edges[bad].tolist()} is shared by "
-                         f"{int(counts[bad])} triangles")
-    if n_tri > 1 and not np.any(counts == 2):
-        raise ValueError(f"nothing is shared: {n_tri} triangles and no common edge; "
-                         f"a soup wants the discontinuous layout")
     local = np.stack([idx[:, [0, 1]], idx[:, [1, 2]], idx[:, [2, 0]]], axis=1).reshape(-1, 2)
     forward = local[:, 0] < local[:, 1]
     n_forward = np.bincount(edge_of.reshape(-1), weights=forward, minlength=edges.shape[0])
-    bad_wind = (counts == 2) & (n_forward != 1)
-    if np.any(bad_wind):
-        bad = int(np.argmax(bad_wind))
-        raise ValueError(f"inconsistent winding: the two triangles on edge "
-                         f"{edges[bad].tolist()} traverse it the same way")
+    _refuse_edges(edges, counts, n_forward, n_tri)
     n_e = edges.shape[0]
     mids = 0.5 * (verts[edges[:, 0]] + verts[edges[:, 1]])
     coords = np.concatenate([verts, mids])
@@ -132,7 +151,7 @@ def shared_nodes(tris, *, groups=None, tol_over_h=WELD_TOL_OVER_H) -> SharedNode
     boundary[n_v + np.flatnonzero(rim)] = True
     boundary[edges[rim].reshape(-1)] = True
     nrm_el = tdcs.unit_normals(tri)
-    area = 0.5 * np.linalg.norm(np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0]), axis=1)
+    area = _area(tri)
     normals = np.zeros((n_v + n_e, 3))
     np.add.at(normals, ids.reshape(-1), np.repeat(area[:, None] * nrm_el, 6, axis=0))
     normals /= np.linalg.norm(normals, axis=1, keepdims=True)
@@ -142,6 +161,72 @@ def shared_nodes(tris, *, groups=None, tol_over_h=WELD_TOL_OVER_H) -> SharedNode
     group[ids.reshape(-1)] = np.repeat(grp, 6)
     return SharedNodes(coords=coords, ids=ids, normals=normals, boundary=boundary,
                        weights=weights, group=group, n_vertices=int(n_v), tol=tol)
+
+
+def shared_nodes_device(tris, *, groups=None, tol_over_h=WELD_TOL_OVER_H) -> SharedNodes:
+    """:func:`shared_nodes` with the weld, the edge table and the per-node
+    sums on the GPU (``mhs_shared_nodes``), for fault networks whose host
+    ``np.unique`` passes dominate: the same arguments, table, refusals and
+    messages.  Node ids, the boundary and the groups equal the host table's
+    exactly; coordinates, normals and weights are summed in the host's order
+    with the same IEEE operations."""
+    import ctypes as C
+
+    from . import _lib, runtime
+    tri, grp, tol = _prepare(tris, groups, tol_over_h)
+    n_tri = tri.shape[0]
+    if not float(np.abs(tri).max()) / tol < 2.0 ** 62:
+        raise ValueError(f"coordinates too large for the weld tolerance {tol:.3e}: "
+                         f"the weld keys would overflow int64")
+    # the element normals as the host table computes them; a collinear
+    # triangle's error is raised where the host raises it, after the refusals
+    try:
+        nrm_el, late = tdcs.unit_normals(tri), None
+    except ValueError as e:
+        nrm_el, late = np.zeros((n_tri, 3)), e
+    area = _area(tri)
+    area_normals = np.ascontiguousarray(area[:, None] * nrm_el)
+    area3 = np.ascontiguousarray(area / 3.0)
+
+    dev = runtime.device()
+    torch, lib = dev.torch, dev.lib
+    n3 = 3 * n_tri
+    work_bytes = int(lib.mhs_weld_workspace_bytes(n_tri))
+    if work_bytes < 0:
+        raise ValueError(f"shared_nodes_device: {n_tri} triangles is out of range (3 n_tri < 2^31)")
+
+    def put(a):
+        return torch.from_numpy(np.ascontiguousarray(a)).to(dev.dev)
+
+    d_tri, d_grp, d_an, d_a3 = put(tri), put(grp), put(area_normals), put(area3)
+    f64, i64 = torch.float64, torch.int64
+    work = torch.empty(work_bytes, dtype=torch.uint8, device=dev.dev)
+    ids = torch.empty((n_tri, 6), dtype=i64, device=dev.dev)
+    coords = torch.empty((2 * n3, 3), dtype=f64, device=dev.dev)
+    normals = torch.empty((2 * n3, 3), dtype=f64, device=dev.dev)
+    weights = torch.empty(2 * n3, dtype=f64, device=dev.dev)
+    group = torch.empty(2 * n3, dtype=i64, device=dev.dev)
+    boundary = torch.empty(2 * n3, dtype=torch.uint8, device=dev.dev)
+    edges = torch.empty((n3, 2), dtype=i64, device=dev.dev)
+    counts = torch.empty(n3, dtype=torch.int32, device=dev.dev)
+    n_forward = torch.empty(n3, dtype=torch.int32, device=dev.dev)
+    n_nodes = (C.c_int64 * 2)()
+    stream = torch.cuda.current_stream(dev.dev)
+    ptr = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+    _lib.check(lib.mhs_shared_nodes(ptr(d_tri), ptr(d_grp), n_tri, tol, ptr(d_an), ptr(d_a3), ptr(work),
+                                    work_bytes, ptr(ids), ptr(coords), ptr(normals), ptr(weights),
+                                    ptr(group), ptr(boundary), ptr(edges), ptr(counts), ptr(n_forward),
+                                    n_nodes, C.c_void_p(stream.cuda_stream)), lib)
+    n_v, n_e = int(n_nodes[0]), int(n_nodes[1])
+    n = n_v + n_e
+    h_ids = ids.cpu().numpy()
+    _refuse_degenerate(h_ids[:, :3], tol)
+    _refuse_edges(edges[:n_e].cpu().numpy(), counts[:n_e].cpu().numpy(), n_forward[:n_e].cpu().numpy(), n_tri)
+    if late is not None:
+        raise late
+    return SharedNodes(coords=coords[:n].cpu().numpy(), ids=h_ids, normals=normals[:n].cpu().numpy(),
+                       boundary=boundary[:n].cpu().numpy().astype(bool), weights=weights[:n].cpu().numpy(),
+                       group=group[:n].cpu().numpy(), n_vertices=n_v, tol=tol)
 
 
 def check_layout(layout: str) -> str:
