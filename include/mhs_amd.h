@@ -118,6 +118,15 @@ int mhs_strain(double* buf, int64_t n_obs, int64_t n_cols, double mu, double lam
 int mhs_source_major(const double* in, double* out, int64_t n_rec, int64_t n_src, int n_a, int n_b,
                      void* stream);
 
+/* Per-triangle geometry of the shared-node table, as shared.py:175-273 and
+ * tdcs.py:49-66 compute it: area_normals (n_tri, 3) = area * upward unit
+ * normal, area3 (n_tri) = area / 3 (device), and into host stats[4]: the
+ * shortest edge, the largest |coordinate|, the number of triangles with a
+ * non-finite coordinate and the number of collinear ones.  stats_dev is 32
+ * bytes of device memory for the reductions.  Synchronises the stream. */
+int mhs_tri_geometry(const double* tris, int64_t n_tri, double* area_normals, double* area3, void* stats_dev,
+                     double* stats, void* stream);
+
 /* Workspace bytes of mhs_shared_nodes for n_tri triangles (-1 if out of
  * range: 1 <= 3 n_tri < 2^31). */
 int64_t mhs_weld_workspace_bytes(int64_t n_tri);
@@ -131,15 +140,20 @@ int64_t mhs_weld_workspace_bytes(int64_t n_tri);
  * m12 m23 m31 node ids), coords / normals (nodes, 3), weights (nodes),
  * group (nodes) int64, boundary (nodes) uint8, edges (3 n_tri, 2) welded
  * vertex pairs, counts and n_forward (3 n_tri) int32 (triangles on the edge,
- * forward traversals), n_nodes[2] = {n_v, n_e} in host memory.  Node ids and
- * the order of every sum equal the host table's.  Refusals (degenerate,
- * non-manifold, inconsistent winding, soup) are the caller's, from ids,
- * edges, counts and n_forward.  Synchronises the stream once. */
+ * forward traversals), n_nodes[2] = {n_v, n_e} and refusals[0] (the
+ * MHS_REFUSE_* bits found) in host memory.  Node ids and the order of every
+ * sum equal the host table's.  The refusal messages are the caller's, from
+ * ids, edges, counts and n_forward, read only when a bit calls for them.
+ * Synchronises the stream twice. */
+#define MHS_REFUSE_DEGENERATE 1     /* two corners of a triangle welded */
+#define MHS_REFUSE_NON_MANIFOLD 2   /* an edge on more than two triangles */
+#define MHS_REFUSE_SHARED 4         /* some edge on exactly two triangles */
+#define MHS_REFUSE_WINDING 8        /* a shared edge traversed the same way twice */
 int mhs_shared_nodes(const double* tris, const int64_t* groups, int64_t n_tri, double tol,
                      const double* area_normals, const double* area3, void* work, int64_t work_bytes,
                      int64_t* ids, double* coords, double* normals, double* weights, int64_t* group,
                      uint8_t* boundary, int64_t* edges, int* counts, int* n_forward, int64_t* n_nodes,
-                     void* stream);
+                     int* refusals, void* stream);
 
 /* Per-material image-kernel coefficients into host memory (mhs_ncoef()
  * doubles): the exact rational image table with nu = lam / (2 (lam + mu)) and

@@ -14,6 +14,7 @@ ABI_VERSION = 3
 
 KIND = {"disp": 0, "stress": 1, "eigen": 2, "traction": 3, "interaction": 4}
 F_FOLD, F_SUBTRACT, F_PARTS, F_MIXED = 1, 2, 4, 8
+REFUSE_DEGENERATE, REFUSE_NON_MANIFOLD, REFUSE_SHARED, REFUSE_WINDING = 1, 2, 4, 8
 STAGE_PAIR, STAGE_Q, STAGE_FOLD = 1, 2, 4
 
 
@@ -40,7 +41,7 @@ _LIB = None
 
 EXPORTS = ("mhs_assemble", "mhs_histogram", "mhs_gather", "mhs_strain", "mhs_fill_material", "mhs_ncoef",
            "mhs_src_stride", "mhs_abi_version", "mhs_last_error", "mhs_fold_item_bytes", "mhs_source_major",
-           "mhs_weld_workspace_bytes", "mhs_shared_nodes")
+           "mhs_weld_workspace_bytes", "mhs_shared_nodes", "mhs_tri_geometry")
 
 
 def load(path=None):
@@ -71,7 +72,11 @@ def load(path=None):
     lib.mhs_weld_workspace_bytes.restype = C.c_int64
     lib.mhs_shared_nodes.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_double, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_int64] + [C.c_void_p] * 9 + [C.POINTER(C.c_int64),
+                                                                                  C.POINTER(C.c_int),
                                                                                   C.c_void_p]
+    lib.mhs_tri_geometry.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.POINTER(C.c_double), C.c_void_p]
+    lib.mhs_tri_geometry.restype = C.c_int
     lib.mhs_shared_nodes.restype = C.c_int
     lib.mhs_fold_item_bytes.argtypes = [C.c_int]
     lib.mhs_fold_item_bytes.restype = C.c_int64

@@ -37,6 +37,86 @@ constexpr int TB = 256;
 
 unsigned blocks_for(int64_t n) { return (unsigned)((n + TB - 1) / TB); }
 
+// OR of the wave's active lanes' bits, at most one atomic per wave, by its
+// first active lane
+__device__ void wave_or(unsigned long long* flags, unsigned bits) {
+  unsigned long long any = 0;
+  for (unsigned b = 1; b <= 8; b <<= 1)
+    if (__ballot((bits & b) != 0)) any |= b;
+  const unsigned long long active = __ballot(1);
+  if (any && (int)(threadIdx.x & 63) == __ffsll((long long)active) - 1 &&
+      (__hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & any) != any)
+    atomicOr(flags, any);   // only waves with a bit not yet set
+}
+
+// Per triangle, as the host table computes them: the shortest edge (edge k
+// = v_k - v_{k-1}), the largest |coordinate|, non-finite and collinear
+// counts reduced into stats[0..3] (doubles as bit patterns: for
+// non-negative values the unsigned order is the numeric order), and the
+// area-weighted upward unit normal and area / 3 (tdcs.py:49-66).
+__global__ void k_tri_geometry(const double* __restrict__ tris, int64_t n_tri, double* __restrict__ an,
+                               double* __restrict__ area3, unsigned long long* __restrict__ stats) {
+  const int64_t t = (int64_t)blockIdx.x * TB + threadIdx.x;
+  double hmin = __longlong_as_double(0x7ff0000000000000LL), amax = 0.0;
+  unsigned long long nonfinite = 0, collinear = 0;
+  if (t < n_tri) {
+    const double* v = tris + 9 * t;
+    bool fin = true;
+    for (int i = 0; i < 9; ++i) {
+      fin = fin && isfinite(v[i]);
+      amax = fmax(amax, fabs(v[i]));
+    }
+    nonfinite = !fin;
+    for (int k = 0; k < 3; ++k) {
+      const double* a = v + 3 * k;
+      const double* b = v + 3 * (k == 0 ? 2 : k - 1);
+      const double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+      hmin = fmin(hmin, sqrt(dx * dx + dy * dy + dz * dz));
+    }
+    const double a0 = v[3] - v[0], a1 = v[4] - v[1], a2 = v[5] - v[2];
+    const double b0 = v[6] - v[0], b1 = v[7] - v[1], b2 = v[8] - v[2];
+    const double c0 = a1 * b2 - a2 * b1, c1 = a2 * b0 - a0 * b2, c2 = a0 * b1 - a1 * b0;
+    const double norm = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+    collinear = !(norm > 0.0);
+    const double area = 0.5 * norm;
+    double n0 = c0 / norm, n1 = c1 / norm, n2 = c2 / norm;
+    if (n2 < 0.0) { n0 = -n0; n1 = -n1; n2 = -n2; }
+    an[3 * t] = area * n0;
+    an[3 * t + 1] = area * n1;
+    an[3 * t + 2] = area * n2;
+    area3[t] = area / 3.0;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    hmin = fmin(hmin, __shfl_down(hmin, off, 64));
+    amax = fmax(amax, __shfl_down(amax, off, 64));
+    nonfinite += __shfl_down(nonfinite, off, 64);
+    collinear += __shfl_down(collinear, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    // only waves that improve on the value seen so far touch the counters
+    const unsigned long long hb = (unsigned long long)__double_as_longlong(hmin);
+    const unsigned long long ab = (unsigned long long)__double_as_longlong(amax);
+    if (hb < __hip_atomic_load(&stats[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(&stats[0], hb);
+    if (ab > __hip_atomic_load(&stats[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(&stats[1], ab);
+    if (nonfinite) atomicAdd(&stats[2], nonfinite);
+    if (collinear) atomicAdd(&stats[3], collinear);
+  }
+}
+
+__global__ void k_zero_u64(unsigned long long* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ void k_stats_init(unsigned long long* __restrict__ stats) {
+  if (threadIdx.x == 0) {
+    stats[0] = 0x7ff0000000000000ULL;   // +inf
+    stats[1] = 0;
+    stats[2] = 0;
+    stats[3] = 0;
+  }
+}
+
 __global__ void k_weld_keys(const double* __restrict__ tris, const int64_t* __restrict__ groups, int64_t n3,
                             double tol, int64_t* __restrict__ kx, int64_t* __restrict__ ky,
                             int64_t* __restrict__ kz, int64_t* __restrict__ kg, int64_t* __restrict__ pos) {
@@ -53,7 +133,8 @@ __global__ void k_weld_keys(const double* __restrict__ tris, const int64_t* __re
 // local edge slots (v0 v1) (v1 v2) (v2 v0) per triangle, as (min, max); the
 // welded vertex ids are columns 0..2 of the (n_tri, 6) id table
 __global__ void k_edge_keys(const int64_t* __restrict__ ids, int64_t n3, int64_t* __restrict__ ka,
-                            int64_t* __restrict__ kb, int* __restrict__ fwd, int64_t* __restrict__ pos) {
+                            int64_t* __restrict__ kb, int* __restrict__ fwd, int64_t* __restrict__ pos,
+                            unsigned long long* __restrict__ flags) {
   const int64_t j = (int64_t)blockIdx.x * TB + threadIdx.x;
   if (j >= n3) return;
   const int64_t t = j / 3, s = j - 3 * t;
@@ -62,6 +143,7 @@ __global__ void k_edge_keys(const int64_t* __restrict__ ids, int64_t n3, int64_t
   kb[j] = a < b ? b : a;
   fwd[j] = a < b;
   pos[j] = j;
+  wave_or(flags, a == b ? (unsigned)MHS_REFUSE_DEGENERATE : 0u);
 }
 
 __global__ void k_gather_i64(const int64_t* __restrict__ col, const int64_t* __restrict__ perm, int64_t n,
@@ -127,7 +209,8 @@ __global__ void k_edge_rows(const int64_t* __restrict__ ka, const int64_t* __res
                             const int* __restrict__ fwd, const int64_t* __restrict__ perm,
                             const int64_t* __restrict__ start, const int64_t* __restrict__ n_e_dev,
                             const double* __restrict__ verts, int64_t n_v, int64_t* __restrict__ edges,
-                            int* __restrict__ counts, int* __restrict__ n_forward, double* __restrict__ coords) {
+                            int* __restrict__ counts, int* __restrict__ n_forward, double* __restrict__ coords,
+                            unsigned long long* __restrict__ flags) {
   const int64_t e = (int64_t)blockIdx.x * TB + threadIdx.x;
   if (e >= *n_e_dev) return;
   const int64_t a = start[e], b = start[e + 1];
@@ -135,11 +218,14 @@ __global__ void k_edge_rows(const int64_t* __restrict__ ka, const int64_t* __res
   const int64_t p = ka[j0], q = kb[j0];
   int nf = 0;
   for (int64_t i = a; i < b; ++i) nf += fwd[perm[i]];
+  const int cnt = (int)(b - a);
   edges[2 * e] = p;
   edges[2 * e + 1] = q;
-  counts[e] = (int)(b - a);
+  counts[e] = cnt;
   n_forward[e] = nf;
   for (int c = 0; c < 3; ++c) coords[3 * (n_v + e) + c] = 0.5 * (verts[3 * p + c] + verts[3 * q + c]);
+  wave_or(flags, (cnt > 2 ? (unsigned)MHS_REFUSE_NON_MANIFOLD : 0u) | (cnt == 2 ? (unsigned)MHS_REFUSE_SHARED : 0u) |
+                     (cnt == 2 && nf != 1 ? (unsigned)MHS_REFUSE_WINDING : 0u));
 }
 
 __global__ void k_vertex_coords(const double* __restrict__ verts, const int64_t* __restrict__ n_v_dev,
@@ -196,7 +282,7 @@ __global__ void k_boundary(const int64_t* __restrict__ edges, const int* __restr
 // workspace layout, in 8-byte words after 256-byte alignment of each part
 struct Work {
   int64_t *k[4], *tmp_key, *tmp_key2, *perm, *perm2, *flag, *scan, *vperm, *vstart, *eperm, *estart, *ka, *kb;
-  int64_t *count_dev;         // [0] = n_v, [1] = n_e
+  int64_t *count_dev;         // [0] = n_v, [1] = n_e, [2] = refusal bits
   int* fwd;
   double* verts;
   void* sort_tmp;
@@ -287,9 +373,9 @@ int mhs_shared_nodes(const double* tris, const int64_t* groups, int64_t n_tri, d
                      const double* area_normals, const double* area3, void* work, int64_t work_bytes,
                      int64_t* ids, double* coords, double* normals, double* weights, int64_t* group,
                      uint8_t* boundary, int64_t* edges, int* counts, int* n_forward, int64_t* n_nodes,
-                     void* stream) {
+                     int* refusals, void* stream) {
   if (!tris || !groups || !area_normals || !area3 || !work || !ids || !coords || !normals || !weights ||
-      !group || !boundary || !edges || !counts || !n_forward || !n_nodes)
+      !group || !boundary || !edges || !counts || !n_forward || !n_nodes || !refusals)
     return mhs_internal_fail("mhs_shared_nodes: null pointer");
   if (n_tri <= 0 || 3 * n_tri > 0x7fffffffLL) return mhs_internal_fail("mhs_shared_nodes: n_tri out of range");
   if (!(tol > 0.0)) return mhs_internal_fail("mhs_shared_nodes: tol must be positive");
@@ -312,7 +398,9 @@ int mhs_shared_nodes(const double* tris, const int64_t* groups, int64_t n_tri, d
   hipLaunchKernelGGL(k_vertex_coords, dim3(blocks_for(3 * n3)), blk, 0, st, w.verts, &w.count_dev[0], coords);
 
   // 2. edges: unique (min, max) vertex-id pairs of the three slots
-  hipLaunchKernelGGL(k_edge_keys, g3, blk, 0, st, ids, n3, w.ka, w.kb, w.fwd, w.perm);
+  unsigned long long* flags = (unsigned long long*)&w.count_dev[2];
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, flags, (int64_t)1);
+  hipLaunchKernelGGL(k_edge_keys, g3, blk, 0, st, ids, n3, w.ka, w.kb, w.fwd, w.perm, flags);
   int64_t* ecols[2] = {w.ka, w.kb};
   if (sort_unique(w, ecols, 2, n3, w.eperm, st)) return -1;
   if (mhs_internal_check(hipMemcpyAsync(&w.count_dev[1], &w.scan[n3 - 1], 8, hipMemcpyDeviceToDevice, st),
@@ -326,7 +414,7 @@ int mhs_shared_nodes(const double* tris, const int64_t* groups, int64_t n_tri, d
   hipLaunchKernelGGL(k_runs, g3, blk, 0, st, w.flag, w.scan, w.eperm, n3, ids, n_v, (int64_t)6, (int64_t)3,
                      w.estart);
   hipLaunchKernelGGL(k_edge_rows, dim3(blocks_for(n_e)), blk, 0, st, w.ka, w.kb, w.fwd, w.eperm, w.estart,
-                     &w.count_dev[1], w.verts, n_v, edges, counts, n_forward, coords);
+                     &w.count_dev[1], w.verts, n_v, edges, counts, n_forward, coords, flags);
 
   // 3. per-node sums and the boundary
   hipLaunchKernelGGL(k_node_sums, dim3(blocks_for(n_v + n_e)), blk, 0, st, w.vperm, w.vstart, w.eperm, w.estart,
@@ -334,8 +422,37 @@ int mhs_shared_nodes(const double* tris, const int64_t* groups, int64_t n_tri, d
   hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(n_v + n_e)), blk, 0, st, boundary, n_v + n_e);
   hipLaunchKernelGGL(k_boundary, dim3(blocks_for(n_e)), blk, 0, st, edges, counts, n_v, n_e, boundary);
   if (mhs_internal_check(hipGetLastError(), "mhs_shared_nodes launch")) return -1;
+  unsigned long long fl = 0;
+  if (mhs_internal_check(hipMemcpyAsync(&fl, flags, 8, hipMemcpyDeviceToHost, st), "refusal flags") ||
+      mhs_internal_check(hipStreamSynchronize(st), "mhs_shared_nodes sync"))
+    return -1;
   n_nodes[0] = n_v;
   n_nodes[1] = n_e;
+  refusals[0] = (int)fl;
+  return 0;
+}
+
+int mhs_tri_geometry(const double* tris, int64_t n_tri, double* area_normals, double* area3, void* stats_dev,
+                     double* stats, void* stream) {
+  if (!tris || !area_normals || !area3 || !stats_dev || !stats) return mhs_internal_fail("mhs_tri_geometry: null pointer");
+  if (n_tri <= 0 || n_tri > 0x7fffffffLL) return mhs_internal_fail("mhs_tri_geometry: n_tri out of range");
+  hipStream_t st = (hipStream_t)stream;
+  unsigned long long* sd = (unsigned long long*)stats_dev;
+  hipLaunchKernelGGL(k_stats_init, dim3(1), dim3(64), 0, st, sd);
+  hipLaunchKernelGGL(k_tri_geometry, dim3(blocks_for(n_tri)), dim3(TB), 0, st, tris, n_tri, area_normals, area3,
+                     sd);
+  if (mhs_internal_check(hipGetLastError(), "mhs_tri_geometry launch")) return -1;
+  unsigned long long h[4];
+  if (mhs_internal_check(hipMemcpyAsync(h, sd, 32, hipMemcpyDeviceToHost, st), "geometry stats") ||
+      mhs_internal_check(hipStreamSynchronize(st), "mhs_tri_geometry sync"))
+    return -1;
+  double d0, d1;
+  memcpy(&d0, &h[0], 8);
+  memcpy(&d1, &h[1], 8);
+  stats[0] = d0;
+  stats[1] = d1;
+  stats[2] = (double)h[2];
+  stats[3] = (double)h[3];
   return 0;
 }
 

@@ -77,22 +77,31 @@ def _edges(idx):
     return edges, inv.reshape(-1, 3), counts
 
 
-def _prepare(tris, groups, tol_over_h):
-    """Validated float64 triangles, int64 groups and the weld tolerance."""
+def _coerce(tris):
     tri = np.ascontiguousarray(np.asarray(tris, dtype=np.float64))
     if tri.ndim != 3 or tri.shape[1:] != (3, 3) or tri.shape[0] == 0:
         raise ValueError(f"tris must have shape (n_tri, 3, 3), got {tri.shape}")
-    if not np.all(np.isfinite(tri)):
-        raise ValueError("tris must be finite")
-    n_tri = tri.shape[0]
+    return tri
+
+
+def _groups(groups, n_tri):
     grp = (np.zeros(n_tri, dtype=np.int64) if groups is None
            else np.asarray(groups).reshape(-1).astype(np.int64))
     if grp.shape[0] != n_tri:
         raise ValueError(f"groups has {grp.shape[0]} entries for {n_tri} triangles")
+    return np.ascontiguousarray(grp)
+
+
+def _prepare(tris, groups, tol_over_h):
+    """Validated float64 triangles, int64 groups and the weld tolerance."""
+    tri = _coerce(tris)
+    if not np.all(np.isfinite(tri)):
+        raise ValueError("tris must be finite")
+    grp = _groups(groups, tri.shape[0])
     h_min = float(np.linalg.norm(tri - np.roll(tri, 1, axis=1), axis=2).min())
     if not h_min > 0.0:
         raise ValueError("a triangle has a zero-length edge")
-    return tri, np.ascontiguousarray(grp), float(tol_over_h) * h_min
+    return tri, grp, float(tol_over_h) * h_min
 
 
 def _refuse_degenerate(idx, tol):
@@ -164,42 +173,50 @@ def shared_nodes(tris, *, groups=None, tol_over_h=WELD_TOL_OVER_H) -> SharedNode
 
 
 def shared_nodes_device(tris, *, groups=None, tol_over_h=WELD_TOL_OVER_H) -> SharedNodes:
-    """:func:`shared_nodes` with the weld, the edge table and the per-node
-    sums on the GPU (``mhs_shared_nodes``), for fault networks whose host
+    """:func:`shared_nodes` built on the GPU, for fault networks whose host
     ``np.unique`` passes dominate: the same arguments, table, refusals and
-    messages.  Node ids, the boundary and the groups equal the host table's
-    exactly; coordinates, normals and weights are summed in the host's order
-    with the same IEEE operations."""
+    messages.  The per-triangle geometry (``mhs_tri_geometry``: shortest
+    edge, element normals and areas), the weld, the edge table, the per-node
+    sums and the refusal checks (``mhs_shared_nodes``) run on the device;
+    the edge table comes back to the host only to word a refusal.  Node ids,
+    the boundary and the groups equal the host table's, and coordinates,
+    normals and weights are summed in the host's order with the same IEEE
+    operations."""
     import ctypes as C
 
     from . import _lib, runtime
-    tri, grp, tol = _prepare(tris, groups, tol_over_h)
+    tri = _coerce(tris)
     n_tri = tri.shape[0]
-    if not float(np.abs(tri).max()) / tol < 2.0 ** 62:
-        raise ValueError(f"coordinates too large for the weld tolerance {tol:.3e}: "
-                         f"the weld keys would overflow int64")
-    # the element normals as the host table computes them; a collinear
-    # triangle's error is raised where the host raises it, after the refusals
-    try:
-        nrm_el, late = tdcs.unit_normals(tri), None
-    except ValueError as e:
-        nrm_el, late = np.zeros((n_tri, 3)), e
-    area = _area(tri)
-    area_normals = np.ascontiguousarray(area[:, None] * nrm_el)
-    area3 = np.ascontiguousarray(area / 3.0)
-
     dev = runtime.device()
     torch, lib = dev.torch, dev.lib
     n3 = 3 * n_tri
     work_bytes = int(lib.mhs_weld_workspace_bytes(n_tri))
     if work_bytes < 0:
         raise ValueError(f"shared_nodes_device: {n_tri} triangles is out of range (3 n_tri < 2^31)")
-
-    def put(a):
-        return torch.from_numpy(np.ascontiguousarray(a)).to(dev.dev)
-
-    d_tri, d_grp, d_an, d_a3 = put(tri), put(grp), put(area_normals), put(area3)
+    stream = torch.cuda.current_stream(dev.dev)
+    h_st = C.c_void_p(stream.cuda_stream)
+    ptr = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
     f64, i64 = torch.float64, torch.int64
+
+    d_tri = torch.from_numpy(tri).to(dev.dev)
+    area_normals = torch.empty((n_tri, 3), dtype=f64, device=dev.dev)
+    area3 = torch.empty(n_tri, dtype=f64, device=dev.dev)
+    stats_dev = torch.empty(4, dtype=i64, device=dev.dev)
+    stats = (C.c_double * 4)()
+    _lib.check(lib.mhs_tri_geometry(ptr(d_tri), n_tri, ptr(area_normals), ptr(area3), ptr(stats_dev), stats,
+                                    h_st), lib)
+    h_min, coord_max, n_nonfinite, n_collinear = stats[0], stats[1], int(stats[2]), int(stats[3])
+    if n_nonfinite:
+        raise ValueError("tris must be finite")
+    grp = _groups(groups, n_tri)
+    if not h_min > 0.0:
+        raise ValueError("a triangle has a zero-length edge")
+    tol = float(tol_over_h) * h_min
+    if not coord_max / tol < 2.0 ** 62:
+        raise ValueError(f"coordinates too large for the weld tolerance {tol:.3e}: "
+                         f"the weld keys would overflow int64")
+
+    d_grp = torch.from_numpy(grp).to(dev.dev)
     work = torch.empty(work_bytes, dtype=torch.uint8, device=dev.dev)
     ids = torch.empty((n_tri, 6), dtype=i64, device=dev.dev)
     coords = torch.empty((2 * n3, 3), dtype=f64, device=dev.dev)
@@ -211,22 +228,35 @@ def shared_nodes_device(tris, *, groups=None, tol_over_h=WELD_TOL_OVER_H) -> Sha
     counts = torch.empty(n3, dtype=torch.int32, device=dev.dev)
     n_forward = torch.empty(n3, dtype=torch.int32, device=dev.dev)
     n_nodes = (C.c_int64 * 2)()
-    stream = torch.cuda.current_stream(dev.dev)
-    ptr = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
-    _lib.check(lib.mhs_shared_nodes(ptr(d_tri), ptr(d_grp), n_tri, tol, ptr(d_an), ptr(d_a3), ptr(work),
+    refusals = C.c_int(0)
+    _lib.check(lib.mhs_shared_nodes(ptr(d_tri), ptr(d_grp), n_tri, tol, ptr(area_normals), ptr(area3), ptr(work),
                                     work_bytes, ptr(ids), ptr(coords), ptr(normals), ptr(weights),
                                     ptr(group), ptr(boundary), ptr(edges), ptr(counts), ptr(n_forward),
-                                    n_nodes, C.c_void_p(stream.cuda_stream)), lib)
+                                    n_nodes, C.byref(refusals), h_st), lib)
     n_v, n_e = int(n_nodes[0]), int(n_nodes[1])
     n = n_v + n_e
-    h_ids = ids.cpu().numpy()
-    _refuse_degenerate(h_ids[:, :3], tol)
-    _refuse_edges(edges[:n_e].cpu().numpy(), counts[:n_e].cpu().numpy(), n_forward[:n_e].cpu().numpy(), n_tri)
-    if late is not None:
-        raise late
-    return SharedNodes(coords=coords[:n].cpu().numpy(), ids=h_ids, normals=normals[:n].cpu().numpy(),
-                       boundary=boundary[:n].cpu().numpy().astype(bool), weights=weights[:n].cpu().numpy(),
-                       group=group[:n].cpu().numpy(), n_vertices=n_v, tol=tol)
+    bits = refusals.value
+    h_ids = _host_i64(runtime, ids)
+    if bits & _lib.REFUSE_DEGENERATE:
+        _refuse_degenerate(h_ids[:, :3], tol)
+        raise RuntimeError("mhs_shared_nodes flagged a degenerate triangle the id table does not show")
+    if (bits & (_lib.REFUSE_NON_MANIFOLD | _lib.REFUSE_WINDING)
+            or (n_tri > 1 and not bits & _lib.REFUSE_SHARED)):
+        _refuse_edges(edges[:n_e].cpu().numpy(), counts[:n_e].cpu().numpy(), n_forward[:n_e].cpu().numpy(), n_tri)
+        raise RuntimeError("mhs_shared_nodes flagged an edge refusal the edge table does not show")
+    if n_collinear:
+        tdcs.unit_normals(tri)          # raises the host's message for the first collinear triangle
+    return SharedNodes(coords=runtime.to_host(coords[:n]), ids=h_ids, normals=runtime.to_host(normals[:n]),
+                       boundary=boundary[:n].cpu().numpy().astype(bool), weights=runtime.to_host(weights[:n]),
+                       group=_host_i64(runtime, group[:n]), n_vertices=n_v, tol=tol)
+
+
+def _host_i64(runtime, t):
+    """An int64 device tensor on the host through the pinned staged copy
+    (bit for bit: the words travel as float64 and are only copied)."""
+    out = np.empty(tuple(t.shape), dtype=np.int64)
+    runtime.to_host(t.view(runtime.device().torch.float64), out.view(np.float64))
+    return out
 
 
 def check_layout(layout: str) -> str:

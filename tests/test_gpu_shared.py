@@ -67,6 +67,9 @@ def test_refusals_match_the_host():
         "weld tolerance": (np.array([[A, B, C], [B, D, C]]), {"tol_over_h": 3.0}),
         "collinear": (np.array([[A, B, 2 * B - A]]), {}),
         "zero-length edge": (np.array([[A, A, C]]), {}),
+        "non-finite": (np.array([[A, B, C + np.nan]]), {}),
+        "groups length": (np.array([[A, B, C], [B, D, C]]), {"groups": [0, 1, 2]}),
+        "non-finite before groups": (np.array([[A, B, C + np.inf]]), {"groups": [0, 1]}),
     }
     for name, (t, kw) in cases.items():
         want = _message(shared_nodes, t, **kw)
