@@ -144,7 +144,7 @@ int64_t mhs_weld_workspace_bytes(int64_t n_tri);
  * MHS_REFUSE_* bits found) in host memory.  Node ids and the order of every
  * sum equal the host table's.  The refusal messages are the caller's, from
  * ids, edges, counts and n_forward, read only when a bit calls for them.
- * Synchronises the stream twice. */
+ * Synchronises the stream four times (key ranges, node counts, refusals). */
 #define MHS_REFUSE_DEGENERATE 1     /* two corners of a triangle welded */
 #define MHS_REFUSE_NON_MANIFOLD 2   /* an edge on more than two triangles */
 #define MHS_REFUSE_SHARED 4         /* some edge on exactly two triangles */

@@ -13,10 +13,14 @@
 // the area-weighted normals and the edge weights are summed in the host's
 // order (no contraction into FMAs: plain IEEE adds, multiplies, divisions).
 //
+// Each column is sorted on its offset from the column's minimum over only
+// the bits its span needs (the group column of a one-fault mesh needs none;
+// vertex ids need log2(n_v)), instead of all 64.
+//
 // Memory: the caller passes every output at its capacity (3 n_tri vertices,
 // 3 n_tri edges) and one workspace of mhs_weld_workspace_bytes(n_tri) bytes;
-// nothing is allocated here.  The call synchronises its stream once, to read
-// the two node counts.
+// nothing is allocated here.  The call synchronises its stream four times,
+// to read the weld keys' ranges, the two node counts and the refusal bits.
 #include <cstdint>
 #include <cstring>
 
@@ -117,17 +121,78 @@ __global__ void k_stats_init(unsigned long long* __restrict__ stats) {
   }
 }
 
+// Order-preserving map of int64 to uint64 (the sign bit flipped), for the
+// key ranges' unsigned atomics.
+__device__ unsigned long long ordered(int64_t x) { return (unsigned long long)x ^ 0x8000000000000000ULL; }
+
 __global__ void k_weld_keys(const double* __restrict__ tris, const int64_t* __restrict__ groups, int64_t n3,
                             double tol, int64_t* __restrict__ kx, int64_t* __restrict__ ky,
-                            int64_t* __restrict__ kz, int64_t* __restrict__ kg, int64_t* __restrict__ pos) {
+                            int64_t* __restrict__ kz, int64_t* __restrict__ kg, int64_t* __restrict__ pos,
+                            unsigned long long* __restrict__ partial) {
   const int64_t j = (int64_t)blockIdx.x * TB + threadIdx.x;
-  if (j >= n3) return;
-  // np.round (half to even) of the IEEE quotient, then the exact conversion
-  kx[j] = (int64_t)rint(tris[3 * j] / tol);
-  ky[j] = (int64_t)rint(tris[3 * j + 1] / tol);
-  kz[j] = (int64_t)rint(tris[3 * j + 2] / tol);
-  kg[j] = groups[j / 3];
-  pos[j] = j;
+  unsigned long long lo[4] = {~0ULL, ~0ULL, ~0ULL, ~0ULL}, hi[4] = {0, 0, 0, 0};
+  if (j < n3) {
+    // np.round (half to even) of the IEEE quotient, then the exact conversion
+    const int64_t k[4] = {(int64_t)rint(tris[3 * j] / tol), (int64_t)rint(tris[3 * j + 1] / tol),
+                          (int64_t)rint(tris[3 * j + 2] / tol), groups[j / 3]};
+    kx[j] = k[0];
+    ky[j] = k[1];
+    kz[j] = k[2];
+    kg[j] = k[3];
+    pos[j] = j;
+    for (int c = 0; c < 4; ++c) lo[c] = hi[c] = ordered(k[c]);
+  }
+  // each column's range: wave, then workgroup minimum / maximum into this
+  // workgroup's partial (k_range_reduce takes the whole range from those)
+  __shared__ unsigned long long red[TB / 64][8];
+  for (int c = 0; c < 4; ++c)
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long a = __shfl_down(lo[c], off, 64), b = __shfl_down(hi[c], off, 64);
+      lo[c] = a < lo[c] ? a : lo[c];
+      hi[c] = b > hi[c] ? b : hi[c];
+    }
+  if ((threadIdx.x & 63) == 0)
+    for (int c = 0; c < 4; ++c) {
+      red[threadIdx.x >> 6][2 * c] = lo[c];
+      red[threadIdx.x >> 6][2 * c + 1] = hi[c];
+    }
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    const int q = threadIdx.x;
+    unsigned long long v = red[0][q];
+    for (int wv = 1; wv < TB / 64; ++wv)
+      v = (q & 1) ? (red[wv][q] > v ? red[wv][q] : v) : (red[wv][q] < v ? red[wv][q] : v);
+    partial[8 * (int64_t)blockIdx.x + q] = v;
+  }
+}
+
+// The weld keys' ranges from the workgroups' partials: range[2c] = minimum,
+// range[2c + 1] = maximum of column c (order-mapped).  One workgroup.
+__global__ void k_range_reduce(const unsigned long long* __restrict__ partial, int64_t n_blocks,
+                               unsigned long long* __restrict__ range) {
+  __shared__ unsigned long long red[TB / 64][8];
+  unsigned long long v[8];
+  for (int q = 0; q < 8; ++q) v[q] = (q & 1) ? 0ULL : ~0ULL;
+  for (int64_t b = threadIdx.x; b < n_blocks; b += TB)
+    for (int q = 0; q < 8; ++q) {
+      const unsigned long long x = partial[8 * b + q];
+      v[q] = (q & 1) ? (x > v[q] ? x : v[q]) : (x < v[q] ? x : v[q]);
+    }
+  for (int q = 0; q < 8; ++q)
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long x = __shfl_down(v[q], off, 64);
+      v[q] = (q & 1) ? (x > v[q] ? x : v[q]) : (x < v[q] ? x : v[q]);
+    }
+  if ((threadIdx.x & 63) == 0)
+    for (int q = 0; q < 8; ++q) red[threadIdx.x >> 6][q] = v[q];
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    const int q = threadIdx.x;
+    unsigned long long x = red[0][q];
+    for (int wv = 1; wv < TB / 64; ++wv)
+      x = (q & 1) ? (red[wv][q] > x ? red[wv][q] : x) : (red[wv][q] < x ? red[wv][q] : x);
+    range[q] = x;
+  }
 }
 
 // local edge slots (v0 v1) (v1 v2) (v2 v0) per triangle, as (min, max); the
@@ -146,10 +211,12 @@ __global__ void k_edge_keys(const int64_t* __restrict__ ids, int64_t n3, int64_t
   wave_or(flags, a == b ? (unsigned)MHS_REFUSE_DEGENERATE : 0u);
 }
 
-__global__ void k_gather_i64(const int64_t* __restrict__ col, const int64_t* __restrict__ perm, int64_t n,
-                             int64_t* __restrict__ out) {
+// sort key of a column in the current order: its offset from the column's
+// minimum (exact in unsigned arithmetic), so the sort needs only the span's bits
+__global__ void k_gather_key(const int64_t* __restrict__ col, const int64_t* __restrict__ perm, int64_t n,
+                             unsigned long long base, unsigned long long* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * TB + threadIdx.x;
-  if (i < n) out[i] = col[perm[i]];
+  if (i < n) out[i] = (unsigned long long)col[perm[i]] - base;
 }
 
 struct Cols {
@@ -282,7 +349,8 @@ __global__ void k_boundary(const int64_t* __restrict__ edges, const int* __restr
 // workspace layout, in 8-byte words after 256-byte alignment of each part
 struct Work {
   int64_t *k[4], *tmp_key, *tmp_key2, *perm, *perm2, *flag, *scan, *vperm, *vstart, *eperm, *estart, *ka, *kb;
-  int64_t *count_dev;         // [0] = n_v, [1] = n_e, [2] = refusal bits
+  int64_t *count_dev;         // [0] = n_v, [1] = n_e, [2] = refusal bits, [8..15] weld key ranges
+  unsigned long long* partial;  // per-workgroup key ranges of k_weld_keys
   int* fwd;
   double* verts;
   void* sort_tmp;
@@ -296,7 +364,8 @@ size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 hipError_t prim_bytes(int64_t n, size_t& bytes) {
   size_t s = 0, c = 0;
   int64_t* none = nullptr;
-  hipError_t e = rocprim::radix_sort_pairs(nullptr, s, none, none, none, none, (unsigned)n, 0u, 64u);
+  unsigned long long* ukey = nullptr;
+  hipError_t e = rocprim::radix_sort_pairs(nullptr, s, ukey, ukey, none, none, (unsigned)n, 0u, 64u);
   if (e != hipSuccess) return e;
   e = rocprim::inclusive_scan(nullptr, c, none, none, (size_t)n, rocprim::plus<int64_t>());
   bytes = s > c ? s : c;
@@ -322,6 +391,7 @@ hipError_t layout(int64_t n_tri, uint8_t* base, Work& w) {
   w.ka = (int64_t*)take(v);
   w.kb = (int64_t*)take(v);
   w.count_dev = (int64_t*)take(256);
+  w.partial = (unsigned long long*)take(align256((size_t)blocks_for(n3) * 64));
   w.fwd = (int*)take(align256((size_t)n3 * 4));
   w.verts = (double*)take(align256((size_t)n3 * 24));
   hipError_t e = prim_bytes(n3, w.sort_bytes);
@@ -331,16 +401,24 @@ hipError_t layout(int64_t n_tri, uint8_t* base, Work& w) {
   return hipSuccess;
 }
 
+// Significant bits of a column's span (0: every key equal, no sort needed).
+int span_bits(unsigned long long span) { return span ? 64 - __builtin_clzll(span) : 0; }
+
 // Stable LSD sort of positions 0..n-1 by the key columns (most significant
-// first in cols[]), leaving the sorted positions in out_perm and the runs'
-// "new key" flags and inclusive scan in w.flag / w.scan.
-int sort_unique(Work& w, int64_t** cols, int ncol, int64_t n, int64_t* out_perm, hipStream_t st) {
+// first in cols[]), each column sorted on its offset from base[c] over its
+// bits[c] significant bits; leaves the sorted positions in out_perm and the
+// runs' "new key" flags and inclusive scan in w.flag / w.scan.
+int sort_unique(Work& w, int64_t** cols, const unsigned long long* base, const int* bits, int ncol, int64_t n,
+                int64_t* out_perm, hipStream_t st) {
   int64_t *cur = w.perm, *nxt = w.perm2;     // w.perm holds 0..n-1 on entry
+  unsigned long long* key_in = (unsigned long long*)w.tmp_key;
+  unsigned long long* key_out = (unsigned long long*)w.tmp_key2;
   for (int c = ncol - 1; c >= 0; --c) {
-    hipLaunchKernelGGL(k_gather_i64, dim3(blocks_for(n)), dim3(TB), 0, st, cols[c], cur, n, w.tmp_key);
+    if (bits[c] == 0) continue;
+    hipLaunchKernelGGL(k_gather_key, dim3(blocks_for(n)), dim3(TB), 0, st, cols[c], cur, n, base[c], key_in);
     size_t bytes = w.sort_bytes;
-    if (mhs_internal_check(rocprim::radix_sort_pairs(w.sort_tmp, bytes, w.tmp_key, w.tmp_key2, cur, nxt,
-                                                     (unsigned)n, 0u, 64u, st),
+    if (mhs_internal_check(rocprim::radix_sort_pairs(w.sort_tmp, bytes, key_in, key_out, cur, nxt, (unsigned)n,
+                                                     0u, (unsigned)bits[c], st),
                            "weld radix sort"))
       return -1;
     int64_t* s = cur; cur = nxt; nxt = s;
@@ -387,8 +465,21 @@ int mhs_shared_nodes(const double* tris, const int64_t* groups, int64_t n_tri, d
   const dim3 g3(blocks_for(n3)), blk(TB);
 
   // 1. weld: unique (kx, ky, kz, group) rows of the corners
-  hipLaunchKernelGGL(k_weld_keys, g3, blk, 0, st, tris, groups, n3, tol, w.k[0], w.k[1], w.k[2], w.k[3], w.perm);
-  if (sort_unique(w, w.k, 4, n3, w.vperm, st)) return -1;
+  unsigned long long* range = (unsigned long long*)&w.count_dev[8];
+  hipLaunchKernelGGL(k_weld_keys, g3, blk, 0, st, tris, groups, n3, tol, w.k[0], w.k[1], w.k[2], w.k[3], w.perm,
+                     w.partial);
+  hipLaunchKernelGGL(k_range_reduce, dim3(1), blk, 0, st, w.partial, (int64_t)g3.x, range);
+  unsigned long long rh[8];
+  if (mhs_internal_check(hipMemcpyAsync(rh, range, 64, hipMemcpyDeviceToHost, st), "weld key ranges") ||
+      mhs_internal_check(hipStreamSynchronize(st), "mhs_shared_nodes sync"))
+    return -1;
+  unsigned long long wbase[4];
+  int wbits[4];
+  for (int c = 0; c < 4; ++c) {
+    wbase[c] = rh[2 * c] ^ 0x8000000000000000ULL;      // the column's minimum, as raw int64 bits
+    wbits[c] = span_bits(rh[2 * c + 1] - rh[2 * c]);
+  }
+  if (sort_unique(w, w.k, wbase, wbits, 4, n3, w.vperm, st)) return -1;
   hipLaunchKernelGGL(k_runs, g3, blk, 0, st, w.flag, w.scan, w.vperm, n3, ids, (int64_t)0, (int64_t)6,
                      (int64_t)0, w.vstart);
   if (mhs_internal_check(hipMemcpyAsync(&w.count_dev[0], &w.scan[n3 - 1], 8, hipMemcpyDeviceToDevice, st),
@@ -396,21 +487,26 @@ int mhs_shared_nodes(const double* tris, const int64_t* groups, int64_t n_tri, d
     return -1;
   hipLaunchKernelGGL(k_vertex_mean, g3, blk, 0, st, tris, w.vperm, w.vstart, &w.count_dev[0], w.verts);
   hipLaunchKernelGGL(k_vertex_coords, dim3(blocks_for(3 * n3)), blk, 0, st, w.verts, &w.count_dev[0], coords);
+  int64_t n_v = 0;
+  if (mhs_internal_check(hipMemcpyAsync(&n_v, &w.count_dev[0], 8, hipMemcpyDeviceToHost, st), "vertex count") ||
+      mhs_internal_check(hipStreamSynchronize(st), "mhs_shared_nodes sync"))
+    return -1;
 
-  // 2. edges: unique (min, max) vertex-id pairs of the three slots
+  // 2. edges: unique (min, max) vertex-id pairs of the three slots, ids in [0, n_v)
   unsigned long long* flags = (unsigned long long*)&w.count_dev[2];
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, flags, (int64_t)1);
   hipLaunchKernelGGL(k_edge_keys, g3, blk, 0, st, ids, n3, w.ka, w.kb, w.fwd, w.perm, flags);
   int64_t* ecols[2] = {w.ka, w.kb};
-  if (sort_unique(w, ecols, 2, n3, w.eperm, st)) return -1;
+  const unsigned long long ebase[2] = {0, 0};
+  const int ebits[2] = {span_bits((unsigned long long)(n_v - 1)), span_bits((unsigned long long)(n_v - 1))};
+  if (sort_unique(w, ecols, ebase, ebits, 2, n3, w.eperm, st)) return -1;
+  int64_t n_e = 0;
+  if (mhs_internal_check(hipMemcpyAsync(&n_e, &w.scan[n3 - 1], 8, hipMemcpyDeviceToHost, st), "edge count") ||
+      mhs_internal_check(hipStreamSynchronize(st), "mhs_shared_nodes sync"))
+    return -1;
   if (mhs_internal_check(hipMemcpyAsync(&w.count_dev[1], &w.scan[n3 - 1], 8, hipMemcpyDeviceToDevice, st),
                          "edge count"))
     return -1;
-  int64_t nn[2];
-  if (mhs_internal_check(hipMemcpyAsync(nn, w.count_dev, 16, hipMemcpyDeviceToHost, st), "node counts") ||
-      mhs_internal_check(hipStreamSynchronize(st), "mhs_shared_nodes sync"))
-    return -1;
-  const int64_t n_v = nn[0], n_e = nn[1];
   hipLaunchKernelGGL(k_runs, g3, blk, 0, st, w.flag, w.scan, w.eperm, n3, ids, n_v, (int64_t)6, (int64_t)3,
                      w.estart);
   hipLaunchKernelGGL(k_edge_rows, dim3(blocks_for(n_e)), blk, 0, st, w.ka, w.kb, w.fwd, w.eperm, w.estart,
