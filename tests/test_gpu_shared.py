@@ -86,3 +86,17 @@ def test_reference_golden_table(small):
     assert relmax(sn.coords, small["sn_coords"]) < 1e-15
     assert relmax(sn.normals, small["sn_normals"]) < 1e-15
     assert relmax(sn.weights, small["sn_weights"]) < 1e-15
+
+
+def test_device_table_drives_the_continuous_layout():
+    """The device-built table passed as ``shared=`` gives the continuous-layout
+    matrices the host-built table gives, bitwise, on a grouped three-fault
+    network (order 2, per-source eps)."""
+    import mhs_amd as M
+    tris, groups, eps = workloads.c4_network(target=300)
+    mat = M.Material(30.0, 30.0)
+    host, dev = shared_nodes(tris, groups=groups), shared_nodes_device(tris, groups=groups)
+    kw = dict(order=2, layout="continuous", source=("strike", "dip"), receiver=("strike", "dip"))
+    a = M.interaction_matrix(tris, mat, eps, shared=host, **kw)
+    b = M.interaction_matrix(tris, mat, eps, shared=dev, **kw)
+    assert a.shape == (host.n, host.n, 2, 2) and np.array_equal(a, b)
