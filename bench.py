@@ -178,6 +178,45 @@ def shared_nodes_timing() -> list:
     return rows
 
 
+def c4_timing(steps: int = 2) -> dict:
+    """The three-fault network's continuous-layout interaction: about 6,000
+    triangles welded per fault into shared nodes, order 2, per-source eps,
+    receiver and source strike / dip, (nodes, nodes, 2, 2) FP64 left in HBM.
+    Wall ms per assembly in FP64 and in the mixed mode (one warm
+    assembly first), pair-evals/s counted as (receiver node, source element
+    node column) pairs, and the mixed result's relmax against FP64."""
+    import torch
+    import mhs_amd as M
+    from mhs_amd import api, runtime, workloads
+    tris, groups, eps = workloads.c4_network()
+    sn = M.shared_nodes(tris, groups=groups)
+    mat = M.Material(30.0, 30.0)
+    out = {"n_tri": int(tris.shape[0]), "nodes": int(sn.n)}
+    res = {}
+    for mode in ("fp64", "mixed"):
+        with M.precision(mode):
+            job = api.plan(M.interaction_matrix, tris, mat, eps, order=2, layout="continuous", shared=sn,
+                           receiver=("strike", "dip"), source=("strike", "dip"))
+        prep = runtime.Prepared(job)
+        with torch.cuda.stream(prep.stream):
+            prep.launch()
+            torch.cuda.synchronize()
+            prep.check_fold()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                prep.launch()
+            torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / steps
+        pairs = int(sn.n) * int(tris.shape[0]) * 6
+        out[mode] = {"ms": ms, "pair_evals_per_s": pairs / (ms / 1e3), "shape": list(prep.final.shape)}
+        res[mode] = prep.final.clone()
+        del prep
+    d = (res["mixed"] - res["fp64"]).abs().max().item() / res["fp64"].abs().max().item()
+    out["mixed_relmax_vs_fp64"] = d
+    out["output_bytes"] = int(res["fp64"].numel() * 8)
+    return out
+
+
 def _progress(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -260,6 +299,8 @@ def main() -> int:
                     help="requests of the served stream (0 = skip it)")
     ap.add_argument("--source-major", action="store_true",
                     help="time the device source-major copy of interaction-shaped matrices and exit")
+    ap.add_argument("--c4", action="store_true",
+                    help="time the three-fault network's continuous-layout order-2 interaction and exit")
     ap.add_argument("--shared-nodes", action="store_true",
                     help="time the shared-node table, host against device, and exit")
     ap.add_argument("--counter-pass", action="store_true",
@@ -276,6 +317,9 @@ def main() -> int:
         return 0
     if args.shared_nodes:
         print(json.dumps({"shared_nodes": shared_nodes_timing()}))
+        return 0
+    if args.c4:
+        print(json.dumps({"c4": c4_timing()}))
         return 0
     if args.source_major:
         print(json.dumps({"source_major": source_major_timing()}))
