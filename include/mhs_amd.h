@@ -23,20 +23,25 @@
 extern "C" {
 #endif
 
-#define MHS_ABI_VERSION 3
+#define MHS_ABI_VERSION 4
 
 #define MHS_GQ_TOTAL 2377760   /* sum of n^2 for n = 1..192 */
 
 /* Output kinds: the five public matrix forms (matrices.py:231-580).
  *   MHS_DISP         (n_obs, 3, n_cols, 3)     disp_matrix
- *   MHS_STRESS       (n_obs, 3, 3, n_cols, 3)  stress/total_stress/elastic_strain
- *   MHS_EIGEN        (n_obs, 3, 3, n_cols, 3)  eigenstress_matrix
+ *   MHS_STRESS       (n_obs, 3, 3, n_cols, 3)  total stress in out; with MHS_SUBTRACT or
+ *                                              MHS_PARTS the eigenstress in out2 (elastic
+ *                                              stress = out - out2 by mhs_subtract, after
+ *                                              any gather: stress/total_stress/elastic_strain)
+ *   MHS_EIGEN        (n_obs, 3, 3, n_cols, 3)  eigenstress_matrix (the stress form's out2)
  *   MHS_TRACTION     (n_obs, 3, n_cols, 3)     traction_matrix
  *   MHS_INTERACTION  (n_obs, n_cols, n_a, n_b) interaction_matrix */
 enum { MHS_DISP = 0, MHS_STRESS = 1, MHS_EIGEN = 2, MHS_TRACTION = 3, MHS_INTERACTION = 4 };
 
 /* Flags: fold surface-reaching sources (fold=True), subtract the eigenstress
- * (subtract_eigenstress=True), also write the eigenstress to out2 (parts=True),
+ * (subtract_eigenstress=True; the stress form writes it to out2 for the
+ * caller's mhs_subtract, the traction and interaction forms subtract it in
+ * the kernel), also write the eigenstress to out2 (parts=True),
  * mixed precision (MHS_MIXED: far-field quadrature points in FP32 with FP64
  * accumulation; near field, fold and eigenstress stay FP64). */
 enum { MHS_FOLD = 1, MHS_SUBTRACT = 2, MHS_PARTS = 4, MHS_MIXED = 8 };
@@ -68,7 +73,8 @@ typedef struct mhs_launch {
                                MHS_GQ_TOTAL long (x1 = u_i, x2 = u_j (1 - u_i),
                                w = w_i w_j (1 - u_i) of the order-n Gauss-Legendre rule) */
   double* out;              /* output rows [obs_lo, obs_hi) in the kind's layout */
-  double* out2;             /* eigenstress part (MHS_PARTS), same layout, or NULL */
+  double* out2;             /* eigenstress part (MHS_STRESS with MHS_SUBTRACT or MHS_PARTS),
+                               same layout, or NULL */
   const int64_t* strad;     /* indices of the straddling (fold) sources, n_strad of them */
   int64_t n_strad;
   int64_t* fold_list;       /* scratch: 2 x fold_cap int64 (receiver, source) pairs */
@@ -106,6 +112,12 @@ int mhs_gather(const double* scratch, double* out, int64_t n_lead, int64_t n_ele
  * far8; folded pairs; Q-family points and pairs; folded mirror near / far12 /
  * far8.  The work model of the roofline figures. */
 int mhs_histogram(const mhs_launch* launch, unsigned long long* counts, void* stream);
+
+/* a[i] -= b[i] for i < n, device FP64: the elastic stress as the finished
+ * total stress minus the finished eigenstress, as matrices.py:287-294 forms
+ * it (sig -= star), so stress_matrix == total_stress_matrix -
+ * eigenstress_matrix bitwise. */
+int mhs_subtract(double* a, const double* b, int64_t n, void* stream);
 
 /* Elastic strain from elastic stress in place (matrices.py:572-579). */
 int mhs_strain(double* buf, int64_t n_obs, int64_t n_cols, double mu, double lam, void* stream);

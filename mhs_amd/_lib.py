@@ -10,7 +10,7 @@ import ctypes as C
 import pathlib
 
 LIB_PATH = pathlib.Path(__file__).resolve().parent / "libmhs_amd.so"
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 KIND = {"disp": 0, "stress": 1, "eigen": 2, "traction": 3, "interaction": 4}
 F_FOLD, F_SUBTRACT, F_PARTS, F_MIXED = 1, 2, 4, 8
@@ -41,6 +41,7 @@ _LIB = None
 
 EXPORTS = ("mhs_assemble", "mhs_histogram", "mhs_gather", "mhs_strain", "mhs_fill_material", "mhs_ncoef",
            "mhs_src_stride", "mhs_abi_version", "mhs_last_error", "mhs_fold_item_bytes", "mhs_source_major",
+           "mhs_subtract",
            "mhs_weld_workspace_bytes", "mhs_shared_nodes", "mhs_tri_geometry")
 
 
@@ -64,6 +65,8 @@ def load(path=None):
     lib.mhs_gather.restype = C.c_int
     lib.mhs_strain.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_void_p]
     lib.mhs_strain.restype = C.c_int
+    lib.mhs_subtract.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
+    lib.mhs_subtract.restype = C.c_int
     lib.mhs_fill_material.argtypes = [C.c_double, C.c_double, C.c_void_p]
     lib.mhs_fill_material.restype = C.c_int
     lib.mhs_source_major.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_int, C.c_int, C.c_void_p]

@@ -162,7 +162,7 @@ __device__ __attribute__((noinline)) void pair_item_call(const Params& pr, int64
 // SIMD: measured at n = 2000 (stress, traction, 3 x 3 interaction) the lists
 // and the higher occupancy were 5-10 % slower for it, and 10-25 % faster
 // for displacement.
-template <int P, int MODE>   // MODE 0: stress family, 1: displacement, 2: eigenstress only
+template <int P, int MODE>   // MODE 0: stress family (eigenstress included), 1: displacement
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == 1 ? MHS_PAIR_WAVES : 1)))
 k_pair(Params pr) {
   constexpr int K = (P + 1) * (P + 2) / 2;
@@ -984,6 +984,13 @@ __global__ void __launch_bounds__(256) k_source_major(const double* __restrict__
   }
 }
 
+// a -= b elementwise (the elastic stress from the finished total and
+// eigenstress).
+__global__ void k_sub(double* __restrict__ a, const double* __restrict__ b, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) a[t] = a[t] - b[t];
+}
+
 __global__ void k_strain(double* __restrict__ buf, int64_t n_obs, int64_t n_cols, double mu, double lam) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n_obs * n_cols * 3; t += stride) {
@@ -1047,7 +1054,6 @@ static int launch_all(const mhs_launch* L, const Params& pr, hipStream_t st) {
     }
     if (stages & MHS_STAGE_PAIR) {
       if (pr.kind == K_DISP) hipLaunchKernelGGL((k_pair<P, 1>), tiles, block, 0, st, pc);
-      else if (pr.kind == K_EIGEN) hipLaunchKernelGGL((k_pair<P, 2>), grid, block, 0, st, pc);
       else hipLaunchKernelGGL((k_pair<P, 0>), grid, block, 0, st, pc);
       if (check(hipGetLastError(), "k_pair launch")) return -1;
     }
@@ -1121,8 +1127,10 @@ int mhs_assemble(const mhs_launch* L, void* stream) {
     return fail("mhs_assemble: interaction needs 1..3 receiver and source components");
   if (L->kind == K_INTERACTION && !L->sb && L->n_b != 3)
     return fail("mhs_assemble: identity source basis needs n_b = 3");
-  if ((L->flags & F_PARTS) && (L->kind != K_STRESS || !L->out2))
-    return fail("mhs_assemble: parts needs the stress kind and a second output");
+  if ((L->flags & F_PARTS) && L->kind != K_STRESS)
+    return fail("mhs_assemble: parts needs the stress kind");
+  if (L->kind == K_STRESS && (L->flags & (F_PARTS | F_SUBTRACT)) && !L->out2)
+    return fail("mhs_assemble: the stress kind with subtract or parts needs out2 for the eigenstress");
   if (L->obs_lo < 0 || L->obs_hi > L->n_obs || L->obs_lo > L->obs_hi)
     return fail("mhs_assemble: bad observer row range");
   const int K = (L->order + 1) * (L->order + 2) / 2;
@@ -1162,6 +1170,13 @@ int mhs_gather(const double* scratch, double* out, int64_t n_lead, int64_t n_ele
   hipLaunchKernelGGL(k_gather, dim3(grid_1d(total)), dim3(256), 0, (hipStream_t)stream, scratch, out,
                      n_lead, n_elem_cols, n_nodes, tail, ptr, idx, sb, n_a, n_b);
   return check(hipGetLastError(), "k_gather launch");
+}
+
+int mhs_subtract(double* a, const double* b, int64_t n, void* stream) {
+  if (!a || !b) return fail("mhs_subtract: null device pointer");
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_sub, dim3(grid_1d(n)), dim3(256), 0, (hipStream_t)stream, a, b, n);
+  return check(hipGetLastError(), "k_sub launch");
 }
 
 int mhs_strain(double* buf, int64_t n_obs, int64_t n_cols, double mu, double lam, void* stream) {

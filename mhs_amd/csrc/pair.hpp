@@ -233,15 +233,21 @@ MHS_INLINE void emit_stress(const Params& pr, int64_t o, int64_t col, int64_t s,
       }
   const int64_t oo = o - pr.out_lo;
   if (pr.kind == K_STRESS || pr.kind == K_EIGEN) {
-    const bool sub = (pr.flags & F_SUBTRACT) != 0;
+    // The stress form keeps the total stress in out and the eigenstress in
+    // out2; the elastic stress is formed afterwards as out - out2, once every
+    // pass has added its part (matrices.py:287-294 subtracts the finished
+    // eigenstress matrix from the finished total).  The eigenstress form
+    // runs this same kernel and stores only E, so the three entry points
+    // agree bitwise.
+    const bool split = (pr.flags & (F_SUBTRACT | F_PARTS)) != 0;
 #pragma unroll
     for (int ab = 0; ab < 9; ++ab)
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
         const int64_t idx = ((oo * 9 + ab) * pr.n_cols + col) * 3 + j;
-        const double val = (pr.kind == K_EIGEN) ? E[ab * 3 + j] : (H[ab * 3 + j] - (sub ? E[ab * 3 + j] : 0.0));
+        const double val = (pr.kind == K_EIGEN) ? E[ab * 3 + j] : H[ab * 3 + j];
         if (add) pr.out[idx] += val; else pr.out[idx] = val;
-        if (!add && (pr.flags & F_PARTS) && pr.kind == K_STRESS) pr.out2[idx] = E[ab * 3 + j];
+        if (!add && split && pr.kind == K_STRESS) pr.out2[idx] = E[ab * 3 + j];
       }
     return;
   }
@@ -302,7 +308,7 @@ MHS_INLINE void emit_disp(const Params& pr, int64_t o, int64_t col, const double
 
 // ------------------------------------------------------------- work items --
 // One (observer row o, element column col) of the pair pass.
-template <int P, int MODE>   // MODE 0: stress family, 1: displacement, 2: eigenstress only
+template <int P, int MODE>   // MODE 0: stress family (eigenstress included), 1: displacement
 MHS_INLINE void pair_item(const Params& pr, int64_t o, int64_t col) {
   constexpr int K = (P + 1) * (P + 2) / 2;
   const int64_t s = col / K;
@@ -317,11 +323,7 @@ MHS_INLINE void pair_item(const Params& pr, int64_t o, int64_t col) {
   } else {
     double H[27];
     double phi_d, phi_m;
-    pair_stress<P>(pr, src, k, ob, band, MODE == 0, H, phi_d, phi_m);
-    if (MODE == 2) {
-#pragma unroll
-      for (int t = 0; t < 27; ++t) H[t] = 0.0;
-    }
+    pair_stress<P>(pr, src, k, ob, band, true, H, phi_d, phi_m);
     emit_stress(pr, o, col, s, H, phi_d, phi_m, src + S_N, src + S_IN, false);
   }
 }

@@ -22,6 +22,8 @@ AUTOTUNE_PAIRS = 1 << 24               # pairs of the sample each candidate is t
 AUTOTUNE_REPEATS = 2                   # timed launches per candidate (the fastest counts)
 autotune = True                         # off: the library default (16) is used
 
+_ALL_STAGES = _lib.STAGE_PAIR | _lib.STAGE_Q | _lib.STAGE_FOLD
+
 PRECISIONS = ("fp64", "mixed")
 _precision = "fp64"
 
@@ -83,6 +85,13 @@ class Job:
     @property
     def K(self) -> int:
         return (self.order + 1) * (self.order + 2) // 2
+
+    @property
+    def split(self) -> bool:
+        """The stress form keeps the eigenstress in a second buffer: for
+        parts=True, and for the elastic stress, formed at the end as the
+        finished total minus the finished eigenstress (matrices.py:287-294)."""
+        return self.kind == "stress" and (self.parts or self.subtract)
 
     @property
     def n_obs(self) -> int:
@@ -204,20 +213,20 @@ class Prepared:
             self.strad_t = dev.put(strad, np.int64) if strad.size else None
             self.coef_t = dev.coef(job.mu, job.lam)
             self.final = torch.empty(job.out_shape(), dtype=torch.float64, device=dev.dev)
-            self.final2 = torch.empty_like(self.final) if job.parts else None
+            self.final2 = torch.empty_like(self.final) if job.split else None
             if cont:
                 elem_shape = list(job.out_shape(n_cols))
                 if job.kind == "interaction":
                     elem_shape[-1] = 3
                 self.elem_shape = elem_shape
-                row_bytes = 8 * int(np.prod(elem_shape[1:])) * (2 if job.parts else 1)
+                row_bytes = 8 * int(np.prod(elem_shape[1:])) * (2 if job.split else 1)
                 self.rows = max(1, min(job.n_obs, ROW_SCRATCH_BYTES // max(row_bytes, 1)))
                 ptr, idx = _csr(job.ids, job.n_nodes)
                 self.ptr_t, self.idx_t = dev.put(ptr, np.int64), dev.put(idx, np.int64)
                 self.sbn_t = dev.put(job.sb) if job.kind == "interaction" else None
                 scr_shape = [min(self.rows, job.n_obs)] + elem_shape[1:]
                 self.scr = torch.empty(scr_shape, dtype=torch.float64, device=dev.dev)
-                self.scr2 = torch.empty_like(self.scr) if job.parts else None
+                self.scr2 = torch.empty_like(self.scr) if job.split else None
             else:
                 self.rows = job.n_obs
             self.fold_cap = max(1, self.rows * self.n_strad)
@@ -338,11 +347,18 @@ class Prepared:
             row_elems = int(np.prod(self.final.shape[1:]))
             out_p = C.c_void_p(self.final.data_ptr() + 8 * lo * row_elems)
             out2_p = (C.c_void_p(self.final2.data_ptr() + 8 * lo * row_elems)
-                      if job.parts else None)
+                      if job.split else None)
         L = self._launch_struct(lo, hi, out_p, out2_p, stages)
         _lib.check(lib.mhs_assemble(C.byref(L), self.handle), lib)
         if self.cont:
             self._gather(lo, hi)
+        if job.split and job.subtract and (stages == 0 or stages == _ALL_STAGES):
+            # elastic = finished total - finished eigenstress, as the reference forms it
+            row_elems = int(np.prod(self.final.shape[1:]))
+            off = 8 * lo * row_elems
+            _lib.check(lib.mhs_subtract(C.c_void_p(self.final.data_ptr() + off),
+                                        C.c_void_p(self.final2.data_ptr() + off),
+                                        (hi - lo) * row_elems, self.handle), lib)
         if job.strain:
             row_elems = int(np.prod(self.final.shape[1:]))
             _lib.check(lib.mhs_strain(C.c_void_p(self.final.data_ptr() + 8 * lo * row_elems), hi - lo,
@@ -430,7 +446,7 @@ def fold_pairs_bound(obs: np.ndarray, recs: np.ndarray, strad: np.ndarray, chunk
 def job_bytes(job: Job) -> int:
     """Device bytes a :class:`Prepared` of ``job`` holds: outputs, the
     continuous layout's element scratch, the fold list and the inputs."""
-    out = 8 * int(np.prod(job.out_shape())) * (2 if job.parts else 1)
+    out = 8 * int(np.prod(job.out_shape())) * (2 if job.split else 1)
     n_cols = job.n_src * job.K
     scratch = 0
     rows = job.n_obs
@@ -438,7 +454,7 @@ def job_bytes(job: Job) -> int:
         shape = list(job.out_shape(n_cols))
         if job.kind == "interaction":
             shape[-1] = 3
-        row_bytes = 8 * int(np.prod(shape[1:])) * (2 if job.parts else 1)
+        row_bytes = 8 * int(np.prod(shape[1:])) * (2 if job.split else 1)
         rows = max(1, min(job.n_obs, ROW_SCRATCH_BYTES // max(row_bytes, 1)))
         scratch = rows * row_bytes
     n_strad = int(np.sum(np.asarray(job.tris)[:, :, 2].max(axis=1) > -3.0 * np.asarray(job.eps)))

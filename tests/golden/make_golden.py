@@ -192,6 +192,46 @@ def case_c3():
     save("c3_rows", **out)
 
 
+def case_c2():
+    """Rows of the 2,000-triangle Gaussian-field fault: stress_matrix at a
+    few observers (deep, mid-depth and two shallow ones inside the fold band
+    of the top row) and row blocks of the default 3 x 3 interaction_matrix
+    (an edge top-row pair, a mid-fault top-row pair in the band, a mid-depth
+    pair and the last pair)."""
+    obs, tris, eps = workloads.c2_stress()
+    mat = mhs.Material(30.0, 30.0)
+    shallow = np.flatnonzero((obs[:, 2] > -1.2) & (np.abs(obs[:, 1]) < 3.0) & (np.abs(obs[:, 0]) < 15.0))
+    sel = np.concatenate([np.arange(4), shallow[:2]])
+    t0 = time.time()
+    # the inputs travel with the rows: the Gaussian field comes from a
+    # Cholesky factor whose last bits depend on the host's BLAS
+    out = {"obs": obs, "tris": tris, "eps": eps, "sel": sel,
+           "stress": mhs.stress_matrix(obs[sel], tris, mat, eps)}
+    print(f"  c2 stress rows {time.time() - t0:.1f}s", flush=True)
+    for lo in (0, 1000, 1010, 1998):
+        t0 = time.time()
+        out[f"inter_{lo}"] = mhs.interaction_matrix(tris, mat, eps, rows=(lo, lo + 2))
+        print(f"  c2 interaction rows {lo} {time.time() - t0:.1f}s", flush=True)
+    save("c2_rows", **out)
+
+
+def case_c4():
+    """Row blocks of the three-fault continuous-layout order-2 interaction at
+    the default receiver (strike, dip, normal) and source (strike, dip):
+    (rows, V + E, 3, 2)."""
+    tris, groups, eps = workloads.c4_network()
+    mat = mhs.Material(30.0, 30.0)
+    sn = mhs.shared_nodes(tris, groups=groups)
+    out = {"tris": tris, "groups": groups, "eps": eps, "n_nodes": np.array(sn.n)}
+    for lo in (0, 6000, sn.n - 2):
+        t0 = time.time()
+        out[f"rows_{lo}"] = mhs.interaction_matrix(tris, mat, eps, order=2, layout="continuous", shared=sn,
+                                                   source=("strike", "dip"), rows=(lo, lo + 2))
+        out["last_lo"] = np.array(lo)
+        print(f"  c4 rows {lo} {time.time() - t0:.1f}s", flush=True)
+    save("c4_rows", **out)
+
+
 def case_table():
     """Fingerprint of the reference's generated image table, read as data:
     sha256 of the canonical JSON of its four record lists."""
@@ -212,7 +252,7 @@ def case_table():
 
 
 CASES = {"table": case_table, "small": case_small, "qorders": case_qorders, "fold": case_fold,
-         "c1": case_c1, "c3": case_c3}
+         "c1": case_c1, "c2": case_c2, "c3": case_c3, "c4": case_c4}
 
 if __name__ == "__main__":
     names = sys.argv[1:] or list(CASES)

@@ -31,7 +31,7 @@ def test_sizes(lib):
     assert lib.mhs_src_stride() == geometry.SRC_STRIDE
     assert lib.mhs_ncoef() == 2167
     from mhs_amd import _lib
-    assert lib.mhs_abi_version() == _lib.ABI_VERSION == 3
+    assert lib.mhs_abi_version() == _lib.ABI_VERSION == 4
     # the launch descriptor mirror has the header's layout: 31 fields, the
     # flattened Gauss rule pointer right after the 1-D tables
     names = [f[0] for f in _lib.Launch._fields_]
