@@ -98,6 +98,22 @@ typedef struct mhs_launch {
  * shared columns are summed afterwards by mhs_gather. */
 int mhs_assemble(const mhs_launch* launch, void* stream);
 
+/* mhs_assemble for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It has no counterpart in the reference.  The fold's work buffers come from
+ * a static device array of the library (one per GPU, about 65 MB, held until
+ * the process ends) instead of the descriptor: fold_list, fold_count,
+ * fold_cap and fold_work are ignored; fold_work_items is the caller's bound
+ * on the folded (receiver, source) pairs of the launch's rows and must not
+ * exceed MHS_STATIC_FOLD_PAIRS (else the call fails with a message: launch
+ * fewer rows per call); fold_err is required when the fold runs and is
+ * cleared by the call (its bits then report this call only).  No hipMalloc,
+ * hipFree or allocating library call.  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (other streams, other host
+ * threads) race and must be serialised by the caller. */
+#define MHS_STATIC_FOLD_PAIRS 3200
+int mhs_assemble_static(const mhs_launch* launch, void* stream);
+
 /* Continuous layout scatter-add (assemble.py:69-93, 328-330), as a gather:
  * out[..., g, ...] = sum over the element columns idx[ptr[g]:ptr[g+1]]
  * (ascending) of scratch[..., c, ...].  With sb (n_nodes x n_b x 3) the

@@ -23,6 +23,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
+
 #include "fold.hpp"
 #include "../../include/mhs_amd.h"
 
@@ -1091,6 +1093,45 @@ static int launch_all(const mhs_launch* L, const Params& pr, hipStream_t st) {
   return 0;
 }
 
+// -------------------------------------------------------- static scratch --
+// The fold's work buffers for callers that cannot allocate
+// (mhs_assemble_static): one file-scope device array per GPU, holding the
+// pair counters, the folded-pair list and the folded pairs' grids for up to
+// MHS_STATIC_FOLD_PAIRS pairs.  Host code never passes the array's name to
+// a kernel (on the host it is a stand-in, not a device address): its device
+// address is looked up once per device with hipGetSymbolAddress, with that
+// device current, and that address is what the kernels receive.
+#define MHS_STR_(x) #x
+#define MHS_STR(x) MHS_STR_(x)
+static constexpr size_t kStaticListOff = 256;   // after the two int32 pair counters
+static constexpr size_t kStaticWorkOff =
+    (kStaticListOff + 16 * (size_t)MHS_STATIC_FOLD_PAIRS + 255) / 256 * 256;
+static constexpr size_t kStaticBytes = kStaticWorkOff + sizeof(FoldItem) * (size_t)MHS_STATIC_FOLD_PAIRS;
+static_assert(kStaticBytes <= (80u << 20), "static scratch stays modest");
+static_assert(alignof(FoldItem) <= 256, "fold items align inside the scratch");
+static __device__ __align__(256) unsigned char scratch[kStaticBytes];
+
+#define MHS_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_ptr[MHS_MAX_DEVICES];
+
+// The current device's copy of the static scratch (looked up on its first
+// use on that device).
+static int static_scratch(unsigned char** out) {
+  int dev = 0;
+  if (check(hipGetDevice(&dev), "mhs_assemble_static: hipGetDevice")) return -1;
+  if (dev < 0 || dev >= MHS_MAX_DEVICES) return fail("mhs_assemble_static: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "mhs_assemble_static: hipGetSymbolAddress")) return -1;
+    if (!p) return fail("mhs_assemble_static: no device address for the static scratch");
+    g_scratch_ptr[dev] = (unsigned char*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
 // ----------------------------------------------------------------- C-ABI --
 extern "C" {
 
@@ -1159,6 +1200,30 @@ int mhs_assemble(const mhs_launch* L, void* stream) {
     case 1: return launch_all<1>(L, pr, st);
     default: return launch_all<2>(L, pr, st);
   }
+}
+
+int mhs_assemble_static(const mhs_launch* L, void* stream) {
+  if (!L) return fail("mhs_assemble_static: null launch descriptor");
+  const bool folds = (L->flags & F_FOLD) && L->n_strad > 0 &&
+                     (L->stages == 0 || (L->stages & MHS_STAGE_FOLD)) && L->kind != K_EIGEN;
+  mhs_launch S = *L;
+  if (folds) {
+    if (!L->fold_err) return fail("mhs_assemble_static: the fold needs fold_err (one device int32, an output)");
+    if (L->fold_work_items < 0 || L->fold_work_items > MHS_STATIC_FOLD_PAIRS)
+      return fail("mhs_assemble_static: the launch's folded (receiver, source) pairs (fold_work_items) exceed "
+                  "the static scratch's " MHS_STR(MHS_STATIC_FOLD_PAIRS) "; launch fewer observer rows per call");
+    unsigned char* base = nullptr;
+    if (static_scratch(&base)) return -1;
+    S.fold_count = (int32_t*)base;
+    S.fold_list = (int64_t*)(base + kStaticListOff);
+    S.fold_work = base + kStaticWorkOff;
+    S.fold_cap = MHS_STATIC_FOLD_PAIRS;
+    S.fold_work_items = MHS_STATIC_FOLD_PAIRS;
+    // the status of this call only (a fill kernel: no memset node under capture)
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, (hipStream_t)stream, L->fold_err, 1);
+    if (check(hipGetLastError(), "mhs_assemble_static: clearing fold_err")) return -1;
+  }
+  return mhs_assemble(&S, stream);
 }
 
 int mhs_gather(const double* scratch, double* out, int64_t n_lead, int64_t n_elem_cols, int64_t n_nodes,
