@@ -325,31 +325,46 @@ def main() -> int:
         print(json.dumps({"source_major": source_major_timing()}))
         return 0
 
-    from mhs_amd import flops
+    from mhs_amd import flops, runtime
     mixed = args.mode == "mixed"
     _progress("timing the other mode")
     prep_o, dt_o, stage_o = _time_mode(not mixed, args.steps, args.warmup)
     ref = prep_o.final.clone()
-    hist = prep_o.histogram()
+    hist_o = prep_o.histogram()
     del prep_o
     _progress(f"timing {args.mode}")
     prep, dt, stage_ms = _time_mode(mixed, args.steps, args.warmup)
+    hist = prep.histogram()
     a, b = (prep.final, ref) if mixed else (ref, prep.final)
-    relerr = float(((a - b).abs().max() / b.abs().max()).item())
-    del prep, ref
+    d = (a - b).abs()
+    relerr = float((d.max() / b.abs().max()).item())
+    # per receiver row: the row's largest error over the row's largest entry
+    row_relerr = float((d.amax(dim=(1, 2, 3)) / b.abs().amax(dim=(1, 2, 3))).max().item())
+    rule_key = prep.tune_key()[:2] + (True,) + prep.tune_key()[3:]      # the mixed run's tuner entry
+    del prep, ref, a, b, d
 
     n = 10_000
     pairs = n * n
     value = pairs / dt
-    fl = flops.flops(hist)
-    q_s = stage_ms["far"] * 1e-3
-    if mixed:
-        f32 = fl["far_fp32_in_mixed"]
-        f64 = fl["far"] - f32
-    else:
-        f32, f64 = 0.0, fl["far"]
-    achieved = (f32 + f64) / q_s / 1e12
-    peak = (f32 + f64) / (f32 / PEAK_TFLOPS["f32"] + f64 / PEAK_TFLOPS["f64"])
+    h_mixed = hist if mixed else hist_o
+    h_fp64 = hist_o if mixed else hist
+    fl = flops.flops(h_fp64)
+    flm = flops.flops_mixed(h_mixed)
+
+    def roofline(kernel, f32, f64, ms):
+        achieved = (f32 + f64) / (ms * 1e-3) / 1e12
+        peak = (f32 + f64) / (f32 / PEAK_TFLOPS["f32"] + f64 / PEAK_TFLOPS["f64"])
+        return {"bound": "valu", "kernel": kernel, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                "frac": achieved / peak, "flop": f32 + f64, "flop_fp32": f32, "ms": ms}
+
+    st_m, st_d = (stage_ms, stage_o) if mixed else (stage_o, stage_ms)
+    roof_mixed = roofline("k_far_mixed (direct + image R + Q far field at the mixed rule, FP32 points)",
+                          flm["far_fp32"], flm["far_fp64"], st_m["far"])
+    roof_fp64 = roofline("k_far (direct + image R + Q far field at the FP64 rule)", 0.0, fl["far"], st_d["far"])
+    near_roofs = [roofline("k_near (closed-form near field, mixed rule's switch)", 0.0, flm["near"], st_m["near"]),
+                  roofline("k_near (closed-form near field, FP64 rule's switch)", 0.0, fl["near"], st_d["near"])]
+    main_roof = roof_mixed if mixed else roof_fp64
+    dev = runtime.device()
     line = {
         "metric": METRIC,
         "value": value,
@@ -367,16 +382,19 @@ def main() -> int:
                                "strike-slip fault 100 x 15 km reaching z=0 (fold active), seeded 5% vertex "
                                "jitter, eps=0.1, Material(30, 30), order 0",
                    "n": n, "pairs": pairs, "mode": args.mode},
-        "roofline": {"bound": "valu", "kernel": f"k_far (direct + image R + Q far field, contracted, {args.mode})",
-                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": _traffic(args.mode)},
+        "roofline": dict(main_roof, traffic=_traffic(args.mode)),
+        "roofline_other_kernels": [roof_fp64 if mixed else roof_mixed] + near_roofs,
         "stage_ms": stage_ms,
         "other_mode": {"mode": "fp64" if mixed else "mixed", "ms_per_step": dt_o * 1e3,
                        "value": pairs / dt_o, "stage_ms": stage_o},
         "mixed_over_fp64": (dt_o / dt) if mixed else (dt / dt_o),
         "mixed_relmax_vs_fp64": relerr,
-        "histogram": hist,
-        "flop": fl,
+        "mixed_row_relmax_vs_fp64": row_relerr,
+        "mixed_rule": {"chosen": runtime._rule_name(dev.rules[rule_key]) if rule_key in dev.rules else "default",
+                       "tuner": dev.rule_table.get(rule_key, {}),
+                       "bound": runtime.RULE_ERROR_BOUND},
+        "histogram": hist if mixed else hist_o,
+        "flop": {"fp64_rule": fl, "mixed_rule": flm},
     }
     if args.serve_requests > 0:
         line["served"] = served_stream(args.serve_requests, args.mode)

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define MHS_ABI_VERSION 4
+#define MHS_ABI_VERSION 5
 
 #define MHS_GQ_TOTAL 2377760   /* sum of n^2 for n = 1..192 */
 
@@ -50,6 +50,25 @@ enum { MHS_FOLD = 1, MHS_SUBTRACT = 2, MHS_PARTS = 4, MHS_MIXED = 8 };
  * R-family or folded mirror + eigenstress), the Q-family and the Boussinesq
  * fold passes add to it.  A partial mask is for timing a pass alone. */
 enum { MHS_STAGE_PAIR = 1, MHS_STAGE_Q = 2, MHS_STAGE_FOLD = 4 };
+
+/* The mixed mode's far-field rule for the contracted interaction (one or two
+ * output components).  With MHS_MIXED a (receiver, source) pair is near --
+ * closed forms in FP64 -- when sqrt(|x - c|^2 + eps^2) <= near_ratio L (the
+ * reference's D* is 10); otherwise each far-field family (direct on the
+ * source triangle; image R and Q on the mirrored one, measured from its
+ * centroid) takes the Gauss order order[b] + (nodal order) of the first
+ * band b with that distance <= band_hi[b] L (the last band takes the rest),
+ * squared for the collapsed triangle rule; image Q pairs that are near keep
+ * the budget law.  near_ratio = 0 selects the library's default rule.  The rule
+ * spends the mixed mode's error budget (1e-6 of the matrix's largest entry)
+ * on fewer points; the all-FP64 mode always uses the reference's rule
+ * (fullspace/defaults.py:20-23, kernels/image.py:249-254). */
+#define MHS_RULE_BANDS 8
+typedef struct mhs_rule {
+  double near_ratio;
+  double band_hi[MHS_RULE_BANDS];   /* ascending */
+  int32_t order[MHS_RULE_BANDS];    /* 1 .. 190 */
+} mhs_rule;
 
 typedef struct mhs_launch {
   int32_t abi_version;      /* MHS_ABI_VERSION */
@@ -88,6 +107,7 @@ typedef struct mhs_launch {
   void* fold_work;          /* scratch: fold_work_items x mhs_fold_item_bytes(order) bytes, the
                                folded pairs' polar grids and centre loads (prepared in parallel) */
   int64_t fold_work_items;  /* capacity in folded pairs; more pairs than this set fold_err bit 2 */
+  mhs_rule rule;            /* MHS_MIXED contracted interaction: the far-field rule */
 } mhs_launch;
 
 /* Assemble one matrix block: replaces the per-source loops of
@@ -123,10 +143,14 @@ int mhs_gather(const double* scratch, double* out, int64_t n_lead, int64_t n_ele
                int64_t n_nodes, int tail, const int64_t* ptr, const int64_t* idx,
                const double* sb, int n_a, int n_b, void* stream);
 
-/* Branch histogram of the launch's pairs (adds into 13 device uint64
- * counters): pairs; direct near / far-12^2 / far-8^2; image near / far12 /
- * far8; folded pairs; Q-family points and pairs; folded mirror near / far12 /
- * far8.  The work model of the roofline figures. */
+/* Branch histogram of the launch's pairs (adds into 24 device uint64
+ * counters): the FP64 rule's pairs; direct near / far-12^2 / far-8^2; image
+ * near / far12 / far8; folded pairs; Q-family points and pairs; folded mirror
+ * near / far12 / far8; then, for MHS_MIXED on the contracted interaction
+ * (else zero), the mixed rule per (row, source) pair: pairs; direct near /
+ * far pairs / far points; image R near / far pairs / far points; Q pairs /
+ * points / chunks of the mixed far pass; R chunks.  The work model of the
+ * roofline figures. */
 int mhs_histogram(const mhs_launch* launch, unsigned long long* counts, void* stream);
 
 /* a[i] -= b[i] for i < n, device FP64: the elastic stress as the finished

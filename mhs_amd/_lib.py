@@ -10,12 +10,22 @@ import ctypes as C
 import pathlib
 
 LIB_PATH = pathlib.Path(__file__).resolve().parent / "libmhs_amd.so"
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 KIND = {"disp": 0, "stress": 1, "eigen": 2, "traction": 3, "interaction": 4}
 F_FOLD, F_SUBTRACT, F_PARTS, F_MIXED = 1, 2, 4, 8
 REFUSE_DEGENERATE, REFUSE_NON_MANIFOLD, REFUSE_SHARED, REFUSE_WINDING = 1, 2, 4, 8
 STAGE_PAIR, STAGE_Q, STAGE_FOLD = 1, 2, 4
+
+
+RULE_BANDS = 8
+
+
+class Rule(C.Structure):
+    """mirror of ``mhs_rule`` (the mixed mode's far-field rule; near_ratio 0:
+    the library's default)"""
+    _fields_ = [("near_ratio", C.c_double), ("band_hi", C.c_double * RULE_BANDS),
+                ("order", C.c_int32 * RULE_BANDS)]
 
 
 class Launch(C.Structure):
@@ -34,6 +44,7 @@ class Launch(C.Structure):
         ("fold_list", C.c_void_p), ("fold_count", C.c_void_p), ("fold_err", C.c_void_p),
         ("fold_cap", C.c_int64), ("stages", C.c_int32), ("far_tiles", C.c_int32),
         ("fold_work", C.c_void_p), ("fold_work_items", C.c_int64),
+        ("rule", Rule),
     ]
 
 

@@ -625,6 +625,272 @@ k_far(Params pr) {
   }
 }
 
+// ------------------------------------------------------ mixed far pass --
+// The mixed mode's far pass of the contracted interaction.  Its rule (Rule)
+// gives far pairs a few Gauss points per family (one to a few dozen, against
+// 64-144 in the FP64 rule), so a pair's point work no longer fills 8 lanes:
+// a lane takes one (receiver row, source) pair and all of its points, in
+// chunks of NPT.  The image kappas are not stored: each chunk forms
+// kappa_k = sum_v F_v B[k][v] from the block's receiver tables in LDS (read
+// by every lane at one address) as the series consumes it, so no per-pair
+// table limits the occupancy.  Points, kappas and series in FP32; point
+// offsets, weights and the sums over points in FP64.
+#ifndef MHS_FARM_WAVES
+#define MHS_FARM_WAVES 3
+#endif
+#ifndef MHS_FARM_NPT
+#define MHS_FARM_NPT 4   // points per chunk of the mixed far pass
+#endif
+
+// kappa of output o, monomial k: sum_v F[o][v] B[o][8 k + v] (B a receiver
+// table in LDS, rows of 8 floats of which 6 are used).
+template <int M, int NOUT>
+struct KapOnFly {
+  const float* B[NOUT];
+  float F[NOUT][6];
+  __device__ __forceinline__ float operator[](int idx) const {
+    const int o = idx / M, k = idx - (idx / M) * M;
+    const float4 lo = *reinterpret_cast<const float4*>(B[o] + 8 * k);
+    const float2 hi = *reinterpret_cast<const float2*>(B[o] + 8 * k + 4);
+    return fmaf(F[o][5], hi.y, fmaf(F[o][4], hi.x, fmaf(F[o][3], lo.w,
+           fmaf(F[o][2], lo.z, fmaf(F[o][1], lo.y, F[o][0] * lo.x)))));
+  }
+};
+
+// One family of one pair over its ng x ng collapsed Gauss points, NPT at a
+// time, summed times wscale into acc[k][t] for every node k (the series of
+// a point is evaluated once for all K nodes).  FAM 0: direct series on the
+// source triangle (kd: the per-output direct coefficients); 1: image R, 2:
+// image Q on the mirrored triangle (kap: kappa on the fly).
+template <int P, int NOUT, int NPT, int FAM, typename KapT>
+__device__ __forceinline__ void mixed_pass(const Params& pr, const double* __restrict__ src, const double ob[3],
+                                           int ng, const float* kd, const KapT& kap, double wscale,
+                                           double (*acc)[NOUT]) {
+  constexpr int K = (P + 1) * (P + 2) / 2;
+  constexpr bool IMG = FAM != 0;
+  const double* v = src + S_V;
+  const double sz = IMG ? -1.0 : 1.0;
+  const double ov[3] = {ob[0] - v[0], ob[1] - v[1], ob[2] - sz * v[2]};
+  const double e1[3] = {v[3] - v[0], v[4] - v[1], sz * (v[5] - v[2])};
+  const double e2[3] = {v[6] - v[0], v[7] - v[1], sz * (v[8] - v[2])};
+  const float eps2 = (float)(src[S_EPS] * src[S_EPS]);
+  const double ar2 = 2.0 * src[S_AREA] * wscale;
+  const size_t toff = (size_t)(ng - 1) * ng * (2 * ng - 1) / 6;
+  const double* qx1 = pr.gq + toff;
+  const double* qx2 = qx1 + MHS_GQ_TOTAL;
+  const double* qw = qx1 + 2 * MHS_GQ_TOTAL;
+  const int npts = ng * ng;
+  for (int base = 0; base < npts; base += NPT) {
+    // compiler-only fence: the kappas are formed inside each chunk, where the
+    // series consumes them, not hoisted out of the loop into registers
+    __asm__ volatile("" ::: "memory");
+    float x[NPT], y[NPT], z[NPT], ir[NPT], iq[NPT], sv[NOUT * NPT], b1[NPT], b2[NPT];
+    double wn[NPT];
+#pragma unroll
+    for (int u = 0; u < NPT; ++u) {
+      const int g0 = base + u;
+      const bool in = g0 < npts;
+      const int g = in ? g0 : 0;
+      const double x1 = qx1[g], x2 = qx2[g];
+      wn[u] = in ? qw[g] * ar2 : 0.0;
+      b1[u] = (float)x1;
+      b2[u] = (float)x2;
+      const float d1 = (float)fma(-x1, e1[0], fma(-x2, e2[0], ov[0]));
+      const float d2 = (float)fma(-x1, e1[1], fma(-x2, e2[1], ov[1]));
+      const float d3 = (float)fma(-x1, e1[2], fma(-x2, e2[2], ov[2]));
+      x[u] = d1; y[u] = d2; z[u] = d3;
+      radii(d1 * d1 + d2 * d2 + d3 * d3 + eps2, d3, FAM == 2, ir[u], iq[u]);
+    }
+    if constexpr (FAM == 0) d_blocked<float, NPT, NOUT>(kd, x, y, z, ir, sv);
+    else if constexpr (FAM == 1) r_blocked_k<float, NPT, NOUT>(kap, x, y, z, ir, iq, sv);
+    else q_blocked_k<float, NPT, NOUT>(kap, x, y, z, ir, iq, sv);
+#pragma unroll
+    for (int u = 0; u < NPT; ++u) {
+#pragma unroll
+      for (int t = 0; t < NOUT; ++t) {
+        if (P == 0) {
+          acc[0][t] = fma(wn[u], (double)sv[t * NPT + u], acc[0][t]);
+        } else {
+          const double val = wn[u] * (double)sv[t * NPT + u];
+          const float l1 = b1[u], l2 = b2[u], l0 = 1.0f - l1 - l2;
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            float nk;
+            if (P == 1) nk = k == 0 ? l0 : k == 1 ? l1 : l2;
+            else if (k < 3) { const float l = k == 0 ? l0 : k == 1 ? l1 : l2; nk = l * (2.0f * l - 1.0f); }
+            else nk = 4.0f * (k == 3 ? l0 * l1 : k == 4 ? l1 * l2 : l2 * l0);
+            acc[k][t] = fma((double)nk, val, acc[k][t]);
+          }
+        }
+      }
+    }
+  }
+}
+
+// A family's points: one point alone (the order-1 rule) without the chunk's
+// padding, else chunks of MHS_FARM_NPT.
+template <int P, int NOUT, int FAM, typename KapT>
+__device__ __forceinline__ void mixed_family(const Params& pr, const double* __restrict__ src, const double ob[3],
+                                             int ng, const float* kd, const KapT& kap, double wscale,
+                                             double (*acc)[NOUT]) {
+  constexpr int NPT = NOUT == 1 ? MHS_FARM_NPT : (MHS_FARM_NPT + 1) / 2;   // two outputs: half the points
+  if (ng == 1) mixed_pass<P, NOUT, 1, FAM>(pr, src, ob, 1, kd, kap, wscale, acc);
+  else mixed_pass<P, NOUT, NPT, FAM>(pr, src, ob, ng, kd, kap, wscale, acc);
+}
+
+template <int P, int NOUT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MHS_FARM_WAVES)))
+k_far_mixed(Params pr) {
+  constexpr int K = (P + 1) * (P + 2) / 2;
+  constexpr int KN = K * NOUT;
+  __shared__ __align__(16) float BQ_s[2][MHS_QM * 8];
+  __shared__ __align__(16) float BR_s[2][MHS_RM * 8];
+  __shared__ double E_s[2][9];
+  __shared__ double res_s[KN > 1 ? 256 * KN : 1];
+  const int tid = threadIdx.x;
+  const int n_a = pr.n_a, n_b = pr.n_b;
+  const int64_t o = pr.obs_lo + blockIdx.y;
+  const double ob[3] = {pr.obs[3 * o], pr.obs[3 * o + 1], pr.obs[3 * o + 2]};
+  const double* nr = pr.nrm + 3 * o;
+  if (tid < n_a * 9) {
+    const int a = tid / 9, i = (tid % 9) / 3, p = tid % 3;
+    const double* rb = pr.rb + ((int64_t)o * n_a + a) * 3;
+    const double rn = rb[0] * nr[0] + rb[1] * nr[1] + rb[2] * nr[2];
+    E_s[a][i * 3 + p] = pr.mu * (rb[i] * nr[p] + rb[p] * nr[i]) + (i == p ? pr.lam * rn : 0.0);
+  }
+  __syncthreads();
+  for (int t = tid; t < n_a * MHS_QM * 6; t += 256) {
+    const int a = t / (MHS_QM * 6), kv = t % (MHS_QM * 6);
+    double acc = 0.0;
+    for (int r = mhs_qb_ptr[kv]; r < mhs_qb_ptr[kv + 1]; ++r)
+      acc = fma(pr.coef[mhs_qb_coef[r]], E_s[a][mhs_qb_ip[r]], acc);
+    BQ_s[a][(kv / 6) * 8 + kv % 6] = (float)acc;
+  }
+  {
+    const double zp[3] = {1.0, ob[2], ob[2] * ob[2]};
+    for (int t = tid; t < n_a * MHS_RM * 6; t += 256) {
+      const int a = t / (MHS_RM * 6), kv = t % (MHS_RM * 6);
+      double acc = 0.0;
+      for (int r = mhs_rb_ptr[kv]; r < mhs_rb_ptr[kv + 1]; ++r)
+        acc = fma(pr.coef[mhs_rb_coef[r]] * zp[mhs_rb_pz[r]], E_s[a][mhs_rb_ip[r]], acc);
+      BR_s[a][(kv / 6) * 8 + kv % 6] = (float)acc;
+    }
+  }
+  __syncthreads();
+  for (int tile = 0; tile < pr.far_tiles; ++tile) {
+    const int64_t s = ((int64_t)blockIdx.x * pr.far_tiles + tile) * 256 + tid;
+    double acc[K][NOUT];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int t = 0; t < NOUT; ++t) acc[k][t] = 0.0;
+    if (s < pr.n_src) {
+      const double* src = pr.src + s * MHS_SRC_STRIDE;
+      const double eps = src[S_EPS];
+      const double L = src[S_L];
+      const double Dd = centroid_distance(src + S_C, ob, eps);
+      const bool dfar = !(Dd <= pr.near_r * L);
+      const bool img = !in_fold_band(src, ob, pr.flags);
+      const double Di = centroid_distance(src + S_IC, ob, eps);
+      const bool ifar = !(Di <= pr.near_r * src[S_IL]);
+      // source half F_b (Hooke on the source plane, slip along frame row b)
+      const double* n = src + S_N;
+      double F[2][6];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        if (b < n_b) {
+          const double* sb = pr.sb + (s * n_b + b) * 3;
+          half_tensor(sb, n, pr.mu, pr.lam, F[b]);
+        } else {
+#pragma unroll
+          for (int v = 0; v < 6; ++v) F[b][v] = 0.0;
+        }
+      }
+      int oa[NOUT], obb[NOUT];
+#pragma unroll
+      for (int t = 0; t < NOUT; ++t) { oa[t] = t / n_b; obb[t] = t - (t / n_b) * n_b; }
+      // the image kappas' source halves, in FP32 (the FP64 ones die with the direct coefficients)
+      KapOnFly<MHS_RM, NOUT> kr;
+      KapOnFly<MHS_QM, NOUT> kq;
+#pragma unroll
+      for (int t = 0; t < NOUT; ++t) {
+        const int a = oa[t] < n_a ? oa[t] : 0;
+        kr.B[t] = BR_s[a];
+        kq.B[t] = BQ_s[a];
+#pragma unroll
+        for (int v = 0; v < 6; ++v) kr.F[t][v] = kq.F[t][v] = (float)F[obb[t]][v];
+      }
+      if (dfar) {
+        float kd[NOUT * MHS_DC];
+#pragma unroll
+        for (int t = 0; t < NOUT; ++t) {
+          const double* Ea = E_s[oa[t] < n_a ? oa[t] : 0];
+          const double* Fv = F[obb[t]];
+          const double Fm[9] = {Fv[0], Fv[3], Fv[4], Fv[3], Fv[1], Fv[5], Fv[4], Fv[5], Fv[2]};
+          double Mm[9];
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+              Mm[i * 3 + j] = Ea[i * 3] * Fm[j] + Ea[i * 3 + 1] * Fm[3 + j] + Ea[i * 3 + 2] * Fm[6 + j];
+          double EF = 0.0;
+#pragma unroll
+          for (int q = 0; q < 9; ++q) EF = fma(Ea[q], Fm[q], EF);
+          const double trE = Ea[0] + Ea[4] + Ea[8], trF = Fm[0] + Fm[4] + Fm[8];
+          const double e2 = eps * eps;
+          const double* rb = pr.rb + ((int64_t)o * n_a + (oa[t] < n_a ? oa[t] : 0)) * 3;
+          double rFn = 0.0;
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+            rFn += rb[i] * (Fm[i * 3] * nr[0] + Fm[i * 3 + 1] * nr[1] + Fm[i * 3 + 2] * nr[2]);
+          float* c = kd + t * MHS_DC;
+          c[0] = (float)Ea[0]; c[1] = (float)Ea[4]; c[2] = (float)Ea[8];
+          c[3] = (float)Ea[1]; c[4] = (float)Ea[2]; c[5] = (float)Ea[5];
+          c[6] = (float)Fv[0]; c[7] = (float)Fv[1]; c[8] = (float)Fv[2];
+          c[9] = (float)Fv[3]; c[10] = (float)Fv[4]; c[11] = (float)Fv[5];
+          c[12] = (float)Mm[0]; c[13] = (float)Mm[4]; c[14] = (float)Mm[8];
+          c[15] = (float)(0.5 * (Mm[1] + Mm[3])); c[16] = (float)(0.5 * (Mm[2] + Mm[6]));
+          c[17] = (float)(0.5 * (Mm[5] + Mm[7]));
+          c[18] = (float)(pr.k3 * EF + trE * trF);
+          c[19] = (float)(-3.0 * trF);
+          c[20] = (float)(-3.0 * trE);
+          c[21] = (float)(pr.k5e * e2 * EF);
+          c[22] = (float)(pr.k7c * e2);
+          c[23] = (float)((pr.flags & F_SUBTRACT) ? pr.k7e * (e2 * e2) * rFn : 0.0);
+          c[24] = (float)pr.k5c;
+        }
+        const int nd = rule_order(pr.rule, Dd, L) + P;
+        mixed_family<P, NOUT, 0>(pr, src, ob, nd, kd, 0, -pr.C1, acc);
+      }
+      if (img) {
+        if (ifar) {
+          const int ni = rule_order(pr.rule, Di, src[S_IL]) + P;
+          mixed_family<P, NOUT, 1>(pr, src, ob, ni, nullptr, kr, 1.0, acc);
+        }
+        // the Q-family has no closed form: near pairs keep the budget law
+        const int nq = ifar ? rule_order(pr.rule, Di, src[S_IL]) + P : q_order_fast<P>(src, ob);
+        mixed_family<P, NOUT, 2>(pr, src, ob, nq, nullptr, kq, 1.0, acc);
+      }
+    }
+    const int64_t oo = o - pr.out_lo;
+    if (KN == 1) {
+      if (s < pr.n_src) pr.out[oo * pr.n_cols + s] = acc[0][0];
+    } else {
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int t = 0; t < NOUT; ++t) res_s[(tid * K + k) * NOUT + t] = acc[k][t];
+      __syncthreads();
+      // the tile's sources are 256 K consecutive columns: one contiguous run of the row
+      const int64_t s0 = ((int64_t)blockIdx.x * pr.far_tiles + tile) * 256;
+      const int64_t n_here = (pr.n_src - s0 < 256 ? pr.n_src - s0 : 256) * KN;
+      double* dst = pr.out + (oo * pr.n_cols + s0 * K) * NOUT;
+      for (int64_t i = tid; i < n_here; i += 256) dst[i] = res_s[i];
+      __syncthreads();
+    }
+  }
+}
+
 // Near parts of the contracted interaction (closed forms, folded mirror),
 // added after k_far.
 
@@ -880,12 +1146,17 @@ k_fold(Params pr, const int64_t* __restrict__ list,
 
 // ------------------------------------------------------------- histogram --
 // Branch histogram of the pair set, with the kernels' own classification
-// (the work model behind the roofline figures).  counts[0..12]:
-// pairs, direct near / far12 / far8, image near / far12 / far8, fold pairs,
-// Q points, Q pairs, mirror near / far12 / far8.
+// (the work model behind the roofline figures).  counts[0..12], the FP64
+// rule: pairs, direct near / far12 / far8, image near / far12 / far8, fold
+// pairs, Q points, Q pairs, mirror near / far12 / far8.  counts[13..23], the
+// mixed rule of the contracted interaction (zero otherwise), per (row,
+// source) pair: pairs, direct near / far pairs / far points, image R near /
+// far pairs / far points, Q pairs / points / chunks of the mixed far pass,
+// R chunks.
+#define MHS_HIST_N 24
 __global__ void k_hist(Params pr, int order, unsigned long long* __restrict__ counts) {
-  __shared__ unsigned long long acc[13];
-  if (threadIdx.x < 13) acc[threadIdx.x] = 0ull;
+  __shared__ unsigned long long acc[MHS_HIST_N];
+  if (threadIdx.x < MHS_HIST_N) acc[threadIdx.x] = 0ull;
   __syncthreads();
   const int K = (order + 1) * (order + 2) / 2;
   const int64_t o = pr.obs_lo + blockIdx.y;
@@ -911,9 +1182,44 @@ __global__ void k_hist(Params pr, int order, unsigned long long* __restrict__ co
       atomicAdd(&acc[8], (unsigned long long)(nq * nq));
       atomicAdd(&acc[9], 1ull);
     }
+    const bool mixed_rule = (pr.flags & F_MIXED) && pr.kind == K_INTERACTION && pr.sb && pr.n_a * pr.n_b <= 2;
+    if (mixed_rule && col == s * K) {
+      // the mixed far pass's branches (k_far_mixed), once per (row, source) pair
+      constexpr int NPT = MHS_FARM_NPT;
+      const int nptf = pr.n_a * pr.n_b == 1 ? NPT : (NPT + 1) / 2;
+      auto chunks = [&](int n) { return n == 1 ? 1 : (n * n + nptf - 1) / nptf; };
+      atomicAdd(&acc[13], 1ull);
+      const double Dd = centroid_distance(src + S_C, ob, eps);
+      if (Dd <= pr.near_r * L) {
+        atomicAdd(&acc[14], 1ull);
+      } else {
+        const int nd = rule_order(pr.rule, Dd, L) + order;
+        atomicAdd(&acc[15], 1ull);
+        atomicAdd(&acc[16], (unsigned long long)(nd * nd));
+      }
+      if (!band) {
+        const double Di = centroid_distance(src + S_IC, ob, eps);
+        const bool ifar = !(Di <= pr.near_r * src[S_IL]);
+        int nq;
+        if (ifar) {
+          const int ni = rule_order(pr.rule, Di, src[S_IL]) + order;
+          atomicAdd(&acc[18], 1ull);
+          atomicAdd(&acc[19], (unsigned long long)(ni * ni));
+          atomicAdd(&acc[23], (unsigned long long)chunks(ni));
+          nq = ni;
+        } else {
+          atomicAdd(&acc[17], 1ull);
+          nq = order == 0 ? q_order_fast<0>(src, ob)
+                          : (order == 1 ? q_order_fast<1>(src, ob) : q_order_fast<2>(src, ob));
+        }
+        atomicAdd(&acc[20], 1ull);
+        atomicAdd(&acc[21], (unsigned long long)(nq * nq));
+        atomicAdd(&acc[22], (unsigned long long)chunks(nq));
+      }
+    }
   }
   __syncthreads();
-  if (threadIdx.x < 13) atomicAdd(&counts[threadIdx.x], acc[threadIdx.x]);
+  if (threadIdx.x < MHS_HIST_N) atomicAdd(&counts[threadIdx.x], acc[threadIdx.x]);
 }
 
 // ---------------------------------------------------------------- gather --
@@ -1007,6 +1313,48 @@ __global__ void k_strain(double* __restrict__ buf, int64_t n_obs, int64_t n_cols
   }
 }
 
+// The mixed mode's default far-field rule: closed forms up to 4 L, then 3 x 3
+// points per family up to 8 L, 2 x 2 up to 60 L and one point beyond (plus
+// the nodal order).  Per family and pair its error is at most ~3e-9 of the
+// largest entry of the strike-slip interaction matrix of a 10,000-triangle
+// fault (measured against the FP64 rule per distance band), within the
+// mixed mode's bound of 1e-6.
+#ifndef MHS_FARM_CT
+#define MHS_FARM_CT 2   // default column tiles (of 256 sources) per block of the mixed far pass
+#endif
+static Rule default_rule() {
+  Rule r;
+  r.near = 4.0;
+  const double hi[MHS_RULE_BANDS] = {8.0, 60.0, 1e300, 1e300, 1e300, 1e300, 1e300, 1e300};
+  const int n[MHS_RULE_BANDS] = {3, 2, 1, 1, 1, 1, 1, 1};
+  for (int b = 0; b < MHS_RULE_BANDS; ++b) { r.hi[b] = hi[b]; r.n[b] = n[b]; }
+  return r;
+}
+
+// The near/far switch and the far-field rule of a launch: the reference's
+// 10 L, or with MHS_MIXED on the contracted interaction the descriptor's
+// rule (checked) or the default.
+static int fill_rule(const mhs_launch* L, Params& pr) {
+  const bool mixed_rule = L->kind == K_INTERACTION && L->sb && L->n_a * L->n_b <= 2 && (L->flags & F_MIXED);
+  pr.near_r = 10.0;
+  pr.rule = default_rule();
+  if (!mixed_rule) return 0;
+  if (L->rule.near_ratio != 0.0) {
+    const mhs_rule& r = L->rule;
+    if (!(r.near_ratio >= 1.0 && r.near_ratio <= 10.0))
+      return fail("mhs_assemble: rule.near_ratio must be in [1, 10] (or 0 for the default rule)");
+    for (int b = 0; b < MHS_RULE_BANDS; ++b) {
+      if (r.order[b] < 1 || r.order[b] > 190) return fail("mhs_assemble: rule orders must be in 1..190");
+      if (b > 0 && !(r.band_hi[b] >= r.band_hi[b - 1])) return fail("mhs_assemble: rule bands must ascend");
+      pr.rule.hi[b] = r.band_hi[b];
+      pr.rule.n[b] = r.order[b];
+    }
+    pr.rule.near = r.near_ratio;
+  }
+  pr.near_r = pr.rule.near;
+  return 0;
+}
+
 // Row-chunked 2-D launches: blockIdx.y is the observer row within a chunk of
 // at most MHS_ROWS_PER_LAUNCH rows and blockIdx.x the column tile, so neither
 // grid dimension nor the work-item count of one dispatch (32-bit in the AQL
@@ -1026,8 +1374,8 @@ static int launch_all(const mhs_launch* L, const Params& pr, hipStream_t st) {
   const bool contracted = pr.kind == K_INTERACTION && pr.sb && nout <= 2;
   const bool mixed = pr.flags & F_MIXED;
   // points per lane (two outputs: half as many)
-  constexpr int N32 = MHS_FAR_NPT32, N64 = MHS_FAR_NPT64;
-  constexpr int N32b = N32 > 1 ? N32 / 2 : 1, N64b = N64 > 1 ? N64 / 2 : 1;
+  constexpr int N64 = MHS_FAR_NPT64;
+  constexpr int N64b = N64 > 1 ? N64 / 2 : 1;
   for (int64_t lo = L->obs_lo; lo < L->obs_hi; lo += MHS_ROWS_PER_LAUNCH) {
     const int64_t rows = (L->obs_hi - lo) < MHS_ROWS_PER_LAUNCH ? (L->obs_hi - lo) : MHS_ROWS_PER_LAUNCH;
     Params pc = pr;
@@ -1041,10 +1389,11 @@ static int launch_all(const mhs_launch* L, const Params& pr, hipStream_t st) {
           const int64_t per = (int64_t)(256 / lpp) * pr.far_tiles;
           return dim3((unsigned)((pr.n_src + per - 1) / per), (unsigned)rows);
         };
-        constexpr int L32 = MHS_FAR_LPP32, L64 = MHS_FAR_LPP64, L32b = MHS_FAR_LPP32B, L64b = MHS_FAR_LPP64B;
-        if (nout == 1 && mixed) hipLaunchKernelGGL((k_far<float, P, 1, N32, L32>), fg(L32), block, 0, st, pc);
+        constexpr int L64 = MHS_FAR_LPP64, L64b = MHS_FAR_LPP64B;
+        // the mixed mode: a lane per pair at the mixed rule's orders
+        if (nout == 1 && mixed) hipLaunchKernelGGL((k_far_mixed<P, 1>), fg(1), block, 0, st, pc);
         else if (nout == 1) hipLaunchKernelGGL((k_far<double, P, 1, N64, L64>), fg(L64), block, 0, st, pc);
-        else if (mixed) hipLaunchKernelGGL((k_far<float, P, 2, N32b, L32b>), fg(L32b), block, 0, st, pc);
+        else if (mixed) hipLaunchKernelGGL((k_far_mixed<P, 2>), fg(1), block, 0, st, pc);
         else hipLaunchKernelGGL((k_far<double, P, 2, N64b, L64b>), fg(L64b), block, 0, st, pc);
         if (check(hipGetLastError(), "k_far launch")) return -1;
       }
@@ -1193,7 +1542,9 @@ int mhs_assemble(const mhs_launch* L, void* stream) {
     pr.k5c = 3.0 * c34 - 9.0;
   }
   if (L->far_tiles < 0 || L->far_tiles > 4096) return fail("mhs_assemble: far_tiles out of range");
-  pr.far_tiles = L->far_tiles ? L->far_tiles : MHS_QC_CT;
+  const bool contracted = L->kind == K_INTERACTION && L->sb && L->n_a * L->n_b <= 2;
+  pr.far_tiles = L->far_tiles ? L->far_tiles : ((contracted && (L->flags & F_MIXED)) ? MHS_FARM_CT : MHS_QC_CT);
+  if (fill_rule(L, pr)) return -1;
   hipStream_t st = (hipStream_t)stream;
   switch (L->order) {
     case 0: return launch_all<0>(L, pr, st);
@@ -1272,6 +1623,8 @@ int mhs_histogram(const mhs_launch* L, unsigned long long* counts, void* stream)
   Params pr{};
   pr.obs = L->obs; pr.src = L->src; pr.n_obs = L->n_obs; pr.n_src = L->n_src; pr.n_cols = L->n_cols;
   pr.obs_lo = L->obs_lo; pr.flags = L->flags;
+  pr.kind = L->kind; pr.sb = L->sb; pr.n_a = L->n_a; pr.n_b = L->n_b;
+  if (fill_rule(L, pr)) return -1;
   const int64_t nblk = (L->n_cols + 255) / 256;
   if (L->obs_hi <= L->obs_lo || nblk <= 0) return 0;
   if (nblk * 256 > 0xffffffffLL) return fail("mhs_histogram: too many columns for one launch");

@@ -9,6 +9,30 @@
 enum Kind { K_DISP = 0, K_STRESS = 1, K_EIGEN = 2, K_TRACTION = 3, K_INTERACTION = 4 };
 enum Flag { F_FOLD = 1, F_SUBTRACT = 2, F_PARTS = 4, F_MIXED = 8 };
 
+// Quadrature rule of the mixed mode's contracted far pass.  A pair is near
+// (closed form) when sqrt(|x - c|^2 + eps^2) <= near L; otherwise its
+// direct, image R and image Q families each take the Gauss order of the
+// first band with that distance <= hi[b] L (direct: the source's centroid;
+// R and Q: the mirrored centroid), plus the nodal order P.  The all-FP64
+// mode keeps the reference's rule (near 10, 12 x 12 to 40, 8 x 8 beyond,
+// the Q budget law) and never reads this.
+#ifndef MHS_RULE_BANDS
+#define MHS_RULE_BANDS 8
+#endif
+struct Rule {
+  double near;
+  double hi[MHS_RULE_BANDS];
+  int n[MHS_RULE_BANDS];
+};
+
+// Gauss order of a family at distance D from a triangle of size L.
+MHS_INLINE int rule_order(const Rule& r, double D, double L) {
+#pragma unroll
+  for (int b = 0; b < MHS_RULE_BANDS - 1; ++b)
+    if (D <= r.hi[b] * L) return r.n[b];
+  return r.n[MHS_RULE_BANDS - 1];
+}
+
 struct Params {
   const double* obs;     // n_obs x 3
   const double* nrm;     // n_obs x 3 (traction, interaction)
@@ -32,6 +56,10 @@ struct Params {
   // A5 coefficient -6(1-nu), A7 coefficient 30(1-nu), eigenstress scale
   // 15/(8 pi C1), and 1 - c34, 3 c34 - 9 with c34 = 3 - 4 nu
   double C1, k5e, k7c, k7e, k3, k5c;
+  // near/far switch of the contracted interaction's passes, in units of L:
+  // 10 (the reference's D* = 10 L), or the mixed rule's near ratio
+  double near_r;
+  Rule rule;             // the mixed mode's far-pass rule
 };
 
 // Fold classification of one (receiver, source) pair (fold.py:84-118): the
@@ -334,8 +362,8 @@ MHS_INLINE void pair_item(const Params& pr, int64_t o, int64_t col) {
 MHS_INLINE int near_parts(const Params& pr, const double* __restrict__ src, const double ob[3]) {
   const double eps = src[S_EPS];
   const bool band = in_fold_band(src, ob, pr.flags);
-  const bool dnear = centroid_distance(src + S_C, ob, eps) <= 10.0 * src[S_L];
-  const bool rnear = !band && centroid_distance(src + S_IC, ob, eps) <= 10.0 * src[S_IL];
+  const bool dnear = centroid_distance(src + S_C, ob, eps) <= pr.near_r * src[S_L];
+  const bool rnear = !band && centroid_distance(src + S_IC, ob, eps) <= pr.near_r * src[S_IL];
   return (dnear ? 1 : 0) | ((band || rnear) ? 2 : 0);
 }
 
@@ -354,8 +382,8 @@ MHS_INLINE void pair_near_item(const Params& pr, int64_t o, int64_t col) {
   const double ob[3] = {pr.obs[3 * o], pr.obs[3 * o + 1], pr.obs[3 * o + 2]};
   const double eps = src[S_EPS];
   const bool band = in_fold_band(src, ob, pr.flags);
-  const bool dnear = centroid_distance(src + S_C, ob, eps) <= 10.0 * src[S_L];
-  const bool rnear = !band && centroid_distance(src + S_IC, ob, eps) <= 10.0 * src[S_IL];
+  const bool dnear = centroid_distance(src + S_C, ob, eps) <= pr.near_r * src[S_L];
+  const bool rnear = !band && centroid_distance(src + S_IC, ob, eps) <= pr.near_r * src[S_IL];
   if (!dnear && !band && !rnear) return;
   const Mat mt{pr.mu, pr.lam, pr.nu};
   const bool sub = (pr.flags & F_SUBTRACT) != 0;

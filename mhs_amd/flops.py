@@ -76,6 +76,38 @@ FLOP_FAR_FP64_PT = 28
 FLOP_FAR_PAIR, FLOP_FAR_KQ, FLOP_FAR_KR, FLOP_FAR_KD = 150, 2004, 864, 120
 
 
+# the mixed mode's far pass (csrc/kernels.hip k_far_mixed, mixed_pass): per
+# point FP64 15 (offsets 2 x 3 FMAs, weight, the accumulation FMA), FP32 the
+# series above (R^2, 1/R and 1/Q included); per (pair, family) kappa formed
+# from the receiver tables, 11 FP32 FLOPs per monomial (Q 167, R 72
+# monomials); per pair FP64 ~100 (classification, source half) and 120 for
+# the direct coefficients of a far pair.
+FLOP_MIX_FP64_PT = 15
+FLOP_MIX_KQ, FLOP_MIX_KR = 167 * 11, 72 * 11
+FLOP_MIX_PAIR = 100
+
+
+def flops_mixed(h: dict) -> dict:
+    """FLOPs of the mixed mode's far pass and near pass (P0, one output) from
+    the mixed-rule counts of a histogram (``m_*``).  kappa is counted once
+    per (pair, family), the algorithm's minimum; the pass forms it once per
+    chunk of points (``m_q_chunks``, ``m_image_chunks``)."""
+    pts = h["m_direct_points"] + h["m_image_points"] + h["m_q_points"]
+    fp64 = h["m_pairs"] * FLOP_MIX_PAIR + h["m_direct_far"] * FLOP_FAR_KD + pts * FLOP_MIX_FP64_PT
+    fp32 = (h["m_direct_points"] * (FLOP_FAR_D_PT - FLOP_FAR_FP64_PT)
+            + h["m_image_points"] * (FLOP_FAR_R_PT - FLOP_FAR_FP64_PT)
+            + h["m_q_points"] * (FLOP_FAR_Q_PT - FLOP_FAR_FP64_PT)
+            + h["m_image_far"] * FLOP_MIX_KR + h["m_q_pairs"] * FLOP_MIX_KQ)
+    near = (h["m_direct_near"] * (FLOP_DIRECT_NEAR + FLOP_DIRECT_LIFT + FLOP_EPILOGUE)
+            + h["m_image_near"] * (FLOP_IMAGE_NEAR + FLOP_IMAGE_LIFT)
+            + h.get("mirror_near", 0) * FLOP_DIRECT_NEAR + h.get("mirror_far12", 0) * 144 * FLOP_DIRECT_PT
+            + h.get("mirror_far8", 0) * 64 * FLOP_DIRECT_PT + h["fold_pairs"] * FLOP_DIRECT_LIFT)
+    kappa_as_run = (h["m_image_chunks"] * FLOP_MIX_KR + h["m_q_chunks"] * FLOP_MIX_KQ
+                    - h["m_image_far"] * FLOP_MIX_KR - h["m_q_pairs"] * FLOP_MIX_KQ)
+    return {"far": float(fp64 + fp32), "far_fp32": float(fp32), "far_fp64": float(fp64),
+            "near": float(near), "kappa_recomputed": float(kappa_as_run)}
+
+
 def _tri_dist(obs, v):
     """Vectorised point-triangle distance, obs (N, 3), v (S, 3, 3) -> (N, S)."""
     a, b, c = v[:, 0], v[:, 1], v[:, 2]

@@ -183,10 +183,10 @@ def emit_blocked_sum(em: Emitter, name, monos, groups, combine):
     by_d = defaultdict(list)
     for k, m in enumerate(monos):
         by_d[m[:3]].append((k, m))
-    em.w("template <typename T, int NPT, int NOUT>")
-    em.w(f"__device__ __forceinline__ void {name}(const T* __restrict__ kap, const T* x, const T* y,")
-    em.w("    const T* z, const T* ir, const T* iq, T* s)")
-    em.w("{")
+    # the body is emitted twice: reading kappa from memory (a pointer), and
+    # from any indexable KapT (``{name}_k``: kappa formed on the fly)
+    out_em = em
+    em = Emitter()
     em.w(f"  T acc[NOUT][NPT][{len(groups)}];")
     em.w("  T xa[NPT], xy[NPT], d[NPT];")
     init = set()
@@ -253,6 +253,20 @@ def emit_blocked_sum(em: Emitter, name, monos, groups, combine):
     em.w("  for (int j = 0; j < NPT; ++j) {")
     combine(em, gidx)
     em.w("  }")
+    body = em.lines
+    em = out_em
+    em.w("template <typename T, int NPT, int NOUT>")
+    em.w(f"__device__ __forceinline__ void {name}(const T* __restrict__ kap, const T* x, const T* y,")
+    em.w("    const T* z, const T* ir, const T* iq, T* s)")
+    em.w("{")
+    em.lines += body
+    em.w("}")
+    em.w()
+    em.w("template <typename T, int NPT, int NOUT, typename KapT>")
+    em.w(f"__device__ __forceinline__ void {name}_k(const KapT& kap, const T* x, const T* y,")
+    em.w("    const T* z, const T* ir, const T* iq, T* s)")
+    em.w("{")
+    em.lines += body
     em.w("}")
     em.w()
 

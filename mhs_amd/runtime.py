@@ -17,10 +17,59 @@ from . import _lib, geometry
 ROW_SCRATCH_BYTES = 4 << 30      # continuous-layout element-major scratch per launch
 FOLD_ITEM_BYTES = 20200          # upper bound of mhs_fold_item_bytes (checked by the ABI tests)
 FAR_TILE_CANDIDATES = (8, 16, 32)      # column tiles per block of the far pass the tuner tries
+FAR_TILE_CANDIDATES_MIXED = (1, 2, 4, 8)   # the same for the mixed far pass (tiles of 256 sources)
 AUTOTUNE_TIE = 0.05                    # candidates this close to the fastest count as tied: the most tiles wins
 AUTOTUNE_PAIRS = 1 << 24               # pairs of the sample each candidate is timed on
 AUTOTUNE_REPEATS = 2                   # timed launches per candidate (the fastest counts)
-autotune = True                         # off: the library default (16) is used
+autotune = True                         # off: the library default is used
+
+
+def tile_candidates(mixed: bool):
+    return FAR_TILE_CANDIDATES_MIXED if mixed else FAR_TILE_CANDIDATES
+
+
+#: The mixed mode's far-field rule for the contracted interaction, as
+#: (near_ratio, ((band_hi, order), ...)) in units of the source's size L:
+#: None lets the autotuner choose among RULE_CANDIDATES (the library's
+#: default when autotuning is off).  See include/mhs_amd.h (mhs_rule).
+mixed_rule = None
+
+#: Candidate rules of the tuner, from the fewest points to the most.  Each
+#: family's error per distance band was measured against the FP64 rule on the
+#: strike-slip interaction of a 10,000-triangle fault: 3e-9 of the largest
+#: entry or less per family for the first, less for the others.
+RULE_CANDIDATES = (
+    (4.0, ((8.0, 3), (60.0, 2), (1e300, 1))),
+    (4.0, ((10.0, 3), (100.0, 2), (1e300, 1))),
+    (5.0, ((12.0, 4), (30.0, 3), (150.0, 2), (1e300, 1))),
+    (6.0, ((16.0, 5), (40.0, 4), (100.0, 3), (1e300, 2))),
+)
+#: The tuner's bound on a candidate's error over the sample rows, relative to
+#: the sample's largest FP64 entry (the mixed mode's stated bound is 1e-6).
+RULE_ERROR_BOUND = 1e-7
+
+
+def _rule_name(rule) -> str:
+    near, bands = rule
+    return f"near {near:g} L; " + ", ".join(f"n={n} to {hi:g} L" if hi < 1e299 else f"n={n} beyond"
+                                           for hi, n in bands)
+
+
+def rule_struct(rule):
+    """A :class:`_lib.Rule` from (near_ratio, [(band_hi, order), ...])."""
+    r = _lib.Rule()
+    if rule is None:
+        return r
+    near, bands = rule
+    if not 1 <= len(bands) <= _lib.RULE_BANDS:
+        raise ValueError(f"a rule has 1..{_lib.RULE_BANDS} bands")
+    bands = list(bands) + [(1e300, bands[-1][1])] * (_lib.RULE_BANDS - len(bands))
+    r.near_ratio = float(near)
+    for b, (hi, n) in enumerate(bands):
+        r.band_hi[b] = float(hi)
+        r.order[b] = int(n)
+    return r
+
 
 _ALL_STAGES = _lib.STAGE_PAIR | _lib.STAGE_Q | _lib.STAGE_FOLD
 
@@ -144,6 +193,8 @@ class Device:
         self._coef = {}
         self.far_tiles = {}     # autotuner cache: (order, outputs, mixed, log2 columns) -> tiles
         self.tune_times = {}    # the same key -> {tiles: ms of the sample's far pass}
+        self.rules = {}         # mixed far-field rule chosen per tune key
+        self.rule_table = {}    # the same key -> {rule: {relmax, ms}} of every candidate
 
     def coef(self, mu: float, lam: float):
         key = (float(mu), float(lam))
@@ -248,6 +299,8 @@ class Prepared:
         self.contracted = (job.kind == "interaction" and not cont and job.sb is not None
                            and job.n_a * job.n_b <= 2)
         self.far_tiles = 0
+        self.rule = rule_struct(mixed_rule)
+        self.rule_tuned = False
 
     def _launch_struct(self, lo, hi, out_p, out2_p, stages):
         job, dev = self.job, self.dev
@@ -261,7 +314,7 @@ class Prepared:
             out=out_p, out2=out2_p, strad=_ptr(self.strad_t), n_strad=self.n_strad,
             fold_list=_ptr(self.fold_list), fold_count=_ptr(self.fold_cnt), fold_err=_ptr(self.fold_err),
             fold_cap=self.fold_cap, stages=stages, far_tiles=self.far_tiles,
-            fold_work=_ptr(self.fold_work), fold_work_items=self.fold_items)
+            fold_work=_ptr(self.fold_work), fold_work_items=self.fold_items, rule=self.rule)
 
     def chunks(self, max_pairs: int | None = None):
         """Observer-row ranges that together cover the job, each at most
@@ -277,6 +330,73 @@ class Prepared:
         return (self.job.order, self.job.n_a * self.job.n_b, bool(self.job.mixed),
                 int(np.log2(max(self.n_cols, 1))))
 
+    def _sample_blocks(self):
+        """About AUTOTUNE_PAIRS pairs in three blocks of rows spread over the
+        matrix (receivers near its ends and its middle see different near/far
+        mixes), or all rows."""
+        n = self.job.n_obs
+        rows = int(min(n, max(1, AUTOTUNE_PAIRS // max(self.n_cols, 1))))
+        if rows >= n:
+            return [(0, n)]
+        per = max(1, rows // 3)
+        return sorted({(max(0, min(n - per, int(f * n) - per // 2)),
+                        max(0, min(n - per, int(f * n) - per // 2)) + per) for f in (1 / 6, 1 / 2, 5 / 6)})
+
+    def tune_rule(self):
+        """Choose the mixed mode's far-field rule (near/far switch and Gauss
+        orders by distance band) under a measured error bound: on the sample
+        rows the FP64 rule's far and near passes are the reference; every
+        candidate of RULE_CANDIDATES is run in the mixed mode on the same
+        rows, its error max|mixed - FP64| / max|FP64| measured and its passes
+        timed (HIP events, after one warm launch); the fastest candidate
+        within RULE_ERROR_BOUND wins (none within: the most conservative).
+        Cached per (order, outputs, size class) on the device with every
+        candidate's error and time.  A rule fixed in ``mixed_rule`` is used
+        untuned.  Synchronises the stream."""
+        if not (self.contracted and self.job.mixed) or self.rule_tuned:
+            return
+        self.rule_tuned = True
+        dev, torch = self.dev, self.dev.torch
+        if mixed_rule is not None or not autotune:
+            self.rule = rule_struct(mixed_rule)
+            return
+        key = self.tune_key()
+        if key in dev.rules:
+            self.rule = rule_struct(dev.rules[key])
+            return
+        blocks = self._sample_blocks()
+        row_elems = int(np.prod(self.final.shape[1:]))
+        stages = _lib.STAGE_PAIR | _lib.STAGE_Q
+        lib = dev.lib
+
+        def run(flags, rule, tiles):
+            for lo, hi in blocks:
+                L = self._launch_struct(lo, hi, C.c_void_p(self.final.data_ptr() + 8 * lo * row_elems),
+                                        None, stages)
+                L.flags, L.rule, L.far_tiles = flags, rule, tiles
+                _lib.check(lib.mhs_assemble(C.byref(L), self.handle), lib)
+
+        table = {}
+        with torch.cuda.stream(self.stream):
+            run(self.flags & ~_lib.F_MIXED, _lib.Rule(), 0)
+            ref = [self.final[lo:hi].clone() for lo, hi in blocks]
+            scale = max(float(r.abs().max()) for r in ref) or 1.0
+            for cand in RULE_CANDIDATES:
+                rs = rule_struct(cand)
+                run(self.flags, rs, 0)
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(self.stream)
+                run(self.flags, rs, 0)
+                e1.record(self.stream)
+                e1.synchronize()
+                err = max(float((self.final[lo:hi] - r).abs().max()) for (lo, hi), r in zip(blocks, ref)) / scale
+                table[cand] = (err, e0.elapsed_time(e1))
+        ok = [c for c, (err, _) in table.items() if err <= RULE_ERROR_BOUND]
+        best = min(ok, key=lambda c: table[c][1]) if ok else RULE_CANDIDATES[-1]
+        dev.rules[key] = best
+        dev.rule_table[key] = {_rule_name(c): {"relmax": e, "ms": t} for c, (e, t) in table.items()}
+        self.rule = rule_struct(best)
+
     def tune(self):
         """Pick the far pass's column tiles per block: each candidate is timed
         (HIP events, after one warm launch, the faster of AUTOTUNE_REPEATS
@@ -288,6 +408,7 @@ class Prepared:
         Synchronises the stream once per candidate of a new key."""
         if not self.contracted or self.far_tiles:
             return self.far_tiles
+        self.tune_rule()
         key = self.tune_key()
         dev, torch = self.dev, self.dev.torch
         if key in dev.far_tiles:
@@ -295,19 +416,10 @@ class Prepared:
             return self.far_tiles
         if not autotune:
             return 0
-        n = self.job.n_obs
-        rows = int(min(n, max(1, AUTOTUNE_PAIRS // max(self.n_cols, 1))))
-        # three blocks of rows spread over the matrix (receivers near its
-        # ends and its middle see different near/far mixes), or all rows
-        if rows >= n:
-            blocks = [(0, n)]
-        else:
-            per = max(1, rows // 3)
-            blocks = sorted({(max(0, min(n - per, int(f * n) - per // 2)),
-                              max(0, min(n - per, int(f * n) - per // 2)) + per) for f in (1 / 6, 1 / 2, 5 / 6)})
+        blocks = self._sample_blocks()
         times = {}
         with torch.cuda.stream(self.stream):
-            for ct in FAR_TILE_CANDIDATES:
+            for ct in tile_candidates(self.job.mixed):
                 self.far_tiles = ct
                 for lo, hi in blocks:
                     self.launch_rows(lo, hi, _lib.STAGE_Q)
@@ -339,6 +451,8 @@ class Prepared:
         job, lib = self.job, self.dev.lib
         if self.contracted and not self.far_tiles:
             self.tune()
+        if self.contracted and job.mixed and not self.rule_tuned:
+            self.tune_rule()
         if self.cont:
             if hi - lo > self.rows:
                 raise ValueError(f"rows [{lo}, {hi}) exceed the element scratch ({self.rows} rows)")
@@ -381,6 +495,8 @@ class Prepared:
         if self.contracted and not self.far_tiles:
             self.tune()
             torch.cuda.synchronize()
+        if self.contracted and self.job.mixed and not self.rule_tuned:
+            self.tune_rule()
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g, stream=self.stream):
             self.launch(stages)
@@ -404,13 +520,16 @@ class Prepared:
 
     HIST_KEYS = ("pairs", "direct_near", "direct_far12", "direct_far8", "image_near", "image_far12",
                  "image_far8", "fold_pairs", "q_points", "q_pairs", "mirror_near", "mirror_far12",
-                 "mirror_far8")
+                 "mirror_far8", "m_pairs", "m_direct_near", "m_direct_far", "m_direct_points",
+                 "m_image_near", "m_image_far", "m_image_points", "m_q_pairs", "m_q_points",
+                 "m_q_chunks", "m_image_chunks")
 
     def histogram(self) -> dict:
-        """Branch counts of every pair, by the kernels' own classification."""
+        """Branch counts of every pair, by the kernels' own classification
+        (the m_* counts: the mixed rule's far pass, per (row, source) pair)."""
         torch, lib = self.dev.torch, self.dev.lib
         with torch.cuda.stream(self.stream):
-            cnt = torch.zeros(13, dtype=torch.int64, device=self.dev.dev)
+            cnt = torch.zeros(len(self.HIST_KEYS), dtype=torch.int64, device=self.dev.dev)
             L = self._launch_struct(0, self.job.n_obs, None, None, 0)
             _lib.check(lib.mhs_histogram(C.byref(L), _ptr(cnt), self.handle), lib)
             vals = cnt.cpu().tolist()

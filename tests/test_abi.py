@@ -31,13 +31,14 @@ def test_sizes(lib):
     assert lib.mhs_src_stride() == geometry.SRC_STRIDE
     assert lib.mhs_ncoef() == 2167
     from mhs_amd import _lib
-    assert lib.mhs_abi_version() == _lib.ABI_VERSION == 4
+    assert lib.mhs_abi_version() == _lib.ABI_VERSION == 5
     # the launch descriptor mirror has the header's layout: 31 fields, the
     # flattened Gauss rule pointer right after the 1-D tables
     names = [f[0] for f in _lib.Launch._fields_]
     assert names.index("gq") == names.index("gl_w") + 1
     assert geometry.GQ_TOTAL == sum(n * n for n in range(1, 193))
-    assert names[-2:] == ["fold_work", "fold_work_items"]
+    assert names[-3:] == ["fold_work", "fold_work_items", "rule"]
+    assert C.sizeof(_lib.Rule) == 8 + 8 * _lib.RULE_BANDS + 4 * _lib.RULE_BANDS
     from mhs_amd import runtime
     for p in range(3):
         assert 0 < lib.mhs_fold_item_bytes(p) <= runtime.FOLD_ITEM_BYTES

@@ -460,7 +460,7 @@ def test_far_tiles_do_not_change_values(M):
             job = api.plan(M.interaction_matrix, tris, M.Material(30.0, 30.0), eps, receiver="strike",
                            source="strike", rows=(4990, 5030))
         outs = []
-        for ct in runtime.FAR_TILE_CANDIDATES:
+        for ct in runtime.tile_candidates(prec == "mixed"):
             prep = runtime.Prepared(job)
             prep.far_tiles = ct
             with torch.cuda.stream(prep.stream):
@@ -470,5 +470,5 @@ def test_far_tiles_do_not_change_values(M):
         assert all(torch.equal(outs[0], o) for o in outs[1:])
         prep = runtime.Prepared(job)
         ct = prep.tune()
-        assert ct in runtime.FAR_TILE_CANDIDATES
+        assert ct in runtime.tile_candidates(prec == "mixed")
         assert prep.dev.far_tiles[prep.tune_key()] == ct
