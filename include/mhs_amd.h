@@ -108,6 +108,11 @@ typedef struct mhs_launch {
                                folded pairs' polar grids and centre loads (prepared in parallel) */
   int64_t fold_work_items;  /* capacity in folded pairs; more pairs than this set fold_err bit 2 */
   mhs_rule rule;            /* MHS_MIXED contracted interaction: the far-field rule */
+  const double* near_tiles; /* optional (NULL): per 256 element columns, 8 doubles: the bounding
+                               box (lo xyz, hi xyz) of the sources' centroids and mirrored
+                               centroids, their largest L, and 1 if any straddles (fold) else
+                               0; lets the contracted near pass skip column steps with no near
+                               pair (no value changes) */
 } mhs_launch;
 
 /* Assemble one matrix block: replaces the per-source loops of

@@ -44,7 +44,7 @@ class Launch(C.Structure):
         ("fold_list", C.c_void_p), ("fold_count", C.c_void_p), ("fold_err", C.c_void_p),
         ("fold_cap", C.c_int64), ("stages", C.c_int32), ("far_tiles", C.c_int32),
         ("fold_work", C.c_void_p), ("fold_work_items", C.c_int64),
-        ("rule", Rule),
+        ("rule", Rule), ("near_tiles", C.c_void_p),
     ]
 
 

@@ -114,14 +114,21 @@ static unsigned grid_1d(int64_t n) {
 // and without mixing the two kinds of work in a wave.
 #define MHS_ROW_TILE 2048
 
-template <typename Cls>
-__device__ __forceinline__ void list_row_tile(int64_t n, int64_t c0, Cls cls, int* list, int& n1, int& n2) {
+// skip(first column of a step) may declare a whole step of 256 columns
+// class 0 without classifying them (the same answer in every lane).
+struct NoSkip {
+  __device__ __forceinline__ bool operator()(int64_t) const { return false; }
+};
+
+template <typename Cls, typename Skip = NoSkip>
+__device__ __forceinline__ void list_row_tile(int64_t n, int64_t c0, Cls cls, int* list, int& n1, int& n2,
+                                              Skip skip = Skip()) {
   __shared__ int cnt_s[2][4];
   const int tid = threadIdx.x, wave = tid >> 6;
   n1 = n2 = 0;
   for (int step = 0; step < MHS_ROW_TILE; step += 256) {
     const int64_t col = c0 + step + tid;
-    const int c = col < n ? cls(col) : 0;
+    const int c = (col < n && !skip(c0 + step)) ? cls(col) : 0;
     const unsigned long long m1 = __ballot(c == 1), m2 = __ballot(c == 2);
     if ((tid & 63) == 0) {
       cnt_s[0][wave] = __popcll(m1);
@@ -727,6 +734,39 @@ __device__ __forceinline__ void mixed_pass(const Params& pr, const double* __res
   }
 }
 
+// Coefficients of the contracted direct series (MHS_DC floats, the layout of
+// d_blocked) for a receiver half E and a source half F (symmetric, order 00
+// 11 22 01 02 12), with the subtracted eigenstress term when eig (rFn = the
+// receiver-frame row . F . receiver normal).
+__device__ __forceinline__ void direct_series_coefs(const Params& pr, const double E[6], const double F[6],
+                                                    double eps, double rFn, bool eig, float* c) {
+  const double Em[9] = {E[0], E[3], E[4], E[3], E[1], E[5], E[4], E[5], E[2]};
+  const double Fm[9] = {F[0], F[3], F[4], F[3], F[1], F[5], F[4], F[5], F[2]};
+  double Mm[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      Mm[i * 3 + j] = Em[i * 3] * Fm[j] + Em[i * 3 + 1] * Fm[3 + j] + Em[i * 3 + 2] * Fm[6 + j];
+  double EF = 0.0;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) EF = fma(Em[q], Fm[q], EF);
+  const double trE = E[0] + E[1] + E[2], trF = F[0] + F[1] + F[2];
+  const double e2 = eps * eps;
+#pragma unroll
+  for (int t = 0; t < 6; ++t) { c[t] = (float)E[t]; c[6 + t] = (float)F[t]; }
+  c[12] = (float)Mm[0]; c[13] = (float)Mm[4]; c[14] = (float)Mm[8];
+  c[15] = (float)(0.5 * (Mm[1] + Mm[3])); c[16] = (float)(0.5 * (Mm[2] + Mm[6]));
+  c[17] = (float)(0.5 * (Mm[5] + Mm[7]));
+  c[18] = (float)(pr.k3 * EF + trE * trF);
+  c[19] = (float)(-3.0 * trF);
+  c[20] = (float)(-3.0 * trE);
+  c[21] = (float)(pr.k5e * e2 * EF);
+  c[22] = (float)(pr.k7c * e2);
+  c[23] = (float)(eig ? pr.k7e * (e2 * e2) * rFn : 0.0);
+  c[24] = (float)pr.k5c;
+}
+
 // A family's points: one point alone (the order-1 rule) without the chunk's
 // padding, else chunks of MHS_FARM_NPT.
 template <int P, int NOUT, int FAM, typename KapT>
@@ -916,10 +956,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MHS_NE
   const int64_t c0 = (int64_t)blockIdx.x * MHS_ROW_TILE;
   const double ob[3] = {pr.obs[3 * o], pr.obs[3 * o + 1], pr.obs[3 * o + 2]};
   int n_img, n_dir;
+  // a step of 256 columns whose sources' centroids and mirrored centroids
+  // all lie farther than the near switch (none straddling) has no near part
+  auto skip = [&](int64_t col0) {
+    if (!pr.near_tiles) return false;
+    const double* t = pr.near_tiles + 8 * (col0 / 256);
+    if (t[7] != 0.0) return false;
+    const double dx = fmax(fmax(t[0] - ob[0], ob[0] - t[3]), 0.0);
+    const double dy = fmax(fmax(t[1] - ob[1], ob[1] - t[4]), 0.0);
+    const double dz = fmax(fmax(t[2] - ob[2], ob[2] - t[5]), 0.0);
+    return sqrt(dx * dx + dy * dy + dz * dz) > pr.near_r * t[6] * (1.0 + 1e-9);
+  };
   list_row_tile(pr.n_cols, c0, [&](int64_t col) {
     const int parts = near_parts(pr, pr.src + (col / K) * MHS_SRC_STRIDE, ob);
     return (parts & 2) ? 1 : parts == 1 ? 2 : 0;
-  }, list_s, n_img, n_dir);
+  }, list_s, n_img, n_dir, skip);
   for (int i = threadIdx.x; i < n_img + n_dir; i += 256) {
     __asm__ volatile("" ::: "memory");   // no loads hoisted out of the pair loop
     const int c = i < n_img ? list_s[i] : list_s[MHS_ROW_TILE + i - n_img];
@@ -999,18 +1050,23 @@ __device__ __attribute__((noinline)) void surface_load_call(const Params& pr, co
 // grid order, so the summation order is fixed), then evaluates the list with
 // every lane busy: at a surface-reaching fault about 92 % of the points are
 // near, and without the list most far-kernel lanes would idle.
+// With the mixed rule (MIX, far points only) the near/far switch is the
+// rule's and a far point's surface load is the contracted direct series at
+// the rule's order in FP32 points (sums in FP64), per slip direction.
 #define MHS_FOLD_LIST 4096
-template <int P, bool STRESS, bool NEAR>
+template <int P, bool STRESS, bool NEAR, bool MIX = false>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(NEAR ? MHS_FOLD_NEAR_WAVES : MHS_FOLD_FAR_WAVES)))
 k_fold(Params pr, const int64_t* __restrict__ list,
                                               const int* __restrict__ count, int64_t cap,
                                               const FoldItem* __restrict__ work, int64_t work_items) {
+  static_assert(!(NEAR && MIX), "the mixed rule changes the far points only");
   constexpr int K = (P + 1) * (P + 2) / 2;
   constexpr int NA = STRESS ? 18 : 9;
   constexpr int GW = sizeof(FoldGrid) / 8;
   static_assert(sizeof(FoldGrid) % 8 == 0, "FoldGrid copies as 8-byte words");
   __shared__ FoldGrid G;
+  __shared__ float kdz_s[MIX ? 3 * MHS_DC : 1];   // the item's direct-series coefficients, per slip
   __shared__ double sh[3];
   __shared__ double red[NA][8];
   __shared__ double u6[6], w6[6], u4[4], w4[4];
@@ -1034,6 +1090,15 @@ k_fold(Params pr, const int64_t* __restrict__ list,
       unsigned long long* gd = reinterpret_cast<unsigned long long*>(&G);
       for (int w = tid; w < GW; w += blockDim.x) gd[w] = gs[w];
     }
+    if (MIX && tid < 3) {
+      // sigma_zz on the plane z = 0 per unit slip e_tid, minus the eigenstress (fold.py:214-228)
+      const double ez[3] = {0.0, 0.0, 1.0};
+      const double em[3] = {tid == 0 ? 1.0 : 0.0, tid == 1 ? 1.0 : 0.0, tid == 2 ? 1.0 : 0.0};
+      double E[6], Fm[6];
+      half_tensor(ez, ez, pr.mu, pr.lam, E);
+      half_tensor(em, src + S_N, pr.mu, pr.lam, Fm);
+      direct_series_coefs(pr, E, Fm, src[S_EPS], Fm[2], true, kdz_s + tid * MHS_DC);
+    }
     __syncthreads();
     for (int k = 0; k < K; ++k) {
       if (tid < 3) sh[tid] = work[item].s0[k][tid];
@@ -1054,7 +1119,7 @@ k_fold(Params pr, const int64_t* __restrict__ list,
             double x, y, w;
             fold_grid_point(G, xh0, xh1, u6, w6, u4, w4, g, x, y, w);
             const double op[3] = {x, y, 0.0};
-            take = (centroid_distance(src + S_C, op, src[S_EPS]) <= 10.0 * src[S_L]) == NEAR;
+            take = (centroid_distance(src + S_C, op, src[S_EPS]) <= pr.near_r * src[S_L]) == NEAR;
           }
           const unsigned long long m = __ballot(take);
           if ((tid & 63) == 0) wcnt_s[wave] = __popcll(m);
@@ -1078,8 +1143,21 @@ k_fold(Params pr, const int64_t* __restrict__ list,
           double x, y, w;
           fold_grid_point(G, xh0, xh1, u6, w6, u4, w4, g, x, y, w);
           double sl[3];
-          if (NEAR) surface_load_call<P>(pr, src, k, x, y, sl);
-          else surface_load_far<P>(pr, src, k, x, y, sl);
+          if constexpr (NEAR) {
+            surface_load_call<P>(pr, src, k, x, y, sl);
+          } else if constexpr (MIX) {
+            const double op[3] = {x, y, 0.0};
+            const int ng = rule_order(pr.rule, centroid_distance(src + S_C, op, src[S_EPS]), src[S_L]) + P;
+            double am[K][3];
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk) am[kk][0] = am[kk][1] = am[kk][2] = 0.0;
+            mixed_family<P, 3, 0>(pr, src, op, ng, kdz_s, 0, -2.0 * pr.C1, am);
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk)
+              if (kk == k) { sl[0] = am[kk][0]; sl[1] = am[kk][1]; sl[2] = am[kk][2]; }
+          } else {
+            surface_load_far<P>(pr, src, k, x, y, sl);
+          }
           sl[0] -= s0; sl[1] -= s1; sl[2] -= s2;
           const double dx = xh0 - x, dy = xh1 - y;
           if (STRESS) {
@@ -1338,6 +1416,7 @@ static int fill_rule(const mhs_launch* L, Params& pr) {
   const bool mixed_rule = L->kind == K_INTERACTION && L->sb && L->n_a * L->n_b <= 2 && (L->flags & F_MIXED);
   pr.near_r = 10.0;
   pr.rule = default_rule();
+  pr.mixed_rule = mixed_rule ? 1 : 0;
   if (!mixed_rule) return 0;
   if (L->rule.near_ratio != 0.0) {
     const mhs_rule& r = L->rule;
@@ -1432,7 +1511,10 @@ static int launch_all(const mhs_launch* L, const Params& pr, hipStream_t st) {
     const dim3 fg(2048);
     if (stress) {
       hipLaunchKernelGGL((k_fold<P, true, true>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items);
-      hipLaunchKernelGGL((k_fold<P, true, false>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items);
+      if (pr.mixed_rule)
+        hipLaunchKernelGGL((k_fold<P, true, false, true>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items);
+      else
+        hipLaunchKernelGGL((k_fold<P, true, false>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items);
     } else {
       hipLaunchKernelGGL((k_fold<P, false, true>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items);
       hipLaunchKernelGGL((k_fold<P, false, false>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items);
@@ -1530,6 +1612,7 @@ int mhs_assemble(const mhs_launch* L, void* stream) {
   pr.coef = L->coef; pr.gl_x = L->gl_x; pr.gl_w = L->gl_w; pr.gq = L->gq; pr.out = L->out; pr.out2 = L->out2;
   pr.n_obs = L->n_obs; pr.n_src = L->n_src; pr.n_cols = L->n_cols; pr.obs_lo = L->obs_lo;
   pr.out_lo = L->obs_lo;
+  pr.near_tiles = L->near_tiles;
   pr.kind = L->kind; pr.flags = L->flags; pr.n_a = L->n_a; pr.n_b = L->n_b;
   pr.mu = L->mu; pr.lam = L->lam; pr.nu = 0.5 * L->lam / (L->lam + L->mu);
   {

@@ -60,6 +60,11 @@ struct Params {
   // 10 (the reference's D* = 10 L), or the mixed rule's near ratio
   double near_r;
   Rule rule;             // the mixed mode's far-pass rule
+  int mixed_rule;        // 1: the launch runs the mixed rule (mixed mode, contracted interaction)
+  // per 256 element columns: bounding box of the sources' centroids and
+  // mirrored centroids (lo xyz, hi xyz), their largest L, any straddling
+  // source (1/0); or null
+  const double* near_tiles;
 };
 
 // Fold classification of one (receiver, source) pair (fold.py:84-118): the

@@ -227,6 +227,30 @@ def _ptr(t):
     return None if t is None else C.c_void_p(t.data_ptr())
 
 
+def near_tile_bounds(recs: np.ndarray, K: int) -> np.ndarray:
+    """Per 256 element columns (n_cols = n_src K): the bounding box of the
+    covered sources' centroids and mirrored centroids (lo xyz, hi xyz), their
+    largest L and whether any straddles (the fold), as (tiles, 8) float64:
+    the near pass skips a step of columns whose box lies beyond the switch."""
+    n_src = recs.shape[0]
+    n_t = (n_src * K + 255) // 256
+    first = (256 * np.arange(n_t)) // K
+    last = np.minimum(n_src, (256 * np.arange(n_t) + 255) // K + 1)
+    c = recs[:, geometry.S_C:geometry.S_C + 3]
+    ic = recs[:, geometry.S_IC:geometry.S_IC + 3]
+    lo_s, hi_s = np.minimum(c, ic), np.maximum(c, ic)
+    L = np.maximum(recs[:, geometry.S_L], recs[:, geometry.S_IL])
+    st = (recs[:, geometry.S_STRADDLE] != 0.0).astype(float)
+    out = np.empty((n_t, 8))
+    for j in range(n_t):
+        sl = slice(first[j], last[j])
+        out[j, 0:3] = lo_s[sl].min(axis=0)
+        out[j, 3:6] = hi_s[sl].max(axis=0)
+        out[j, 6] = L[sl].max()
+        out[j, 7] = st[sl].max()
+    return out
+
+
 def _csr(ids: np.ndarray, n_nodes: int):
     """Element columns of every shared node, ascending: (ptr, idx)."""
     flat = ids.reshape(-1)
@@ -298,6 +322,10 @@ class Prepared:
         # tunable launch shape: column tiles per block
         self.contracted = (job.kind == "interaction" and not cont and job.sb is not None
                            and job.n_a * job.n_b <= 2)
+        self.near_tiles_t = None
+        if self.contracted:
+            with torch.cuda.stream(self.stream):
+                self.near_tiles_t = dev.put(near_tile_bounds(recs, K))
         self.far_tiles = 0
         self.rule = rule_struct(mixed_rule)
         self.rule_tuned = False
@@ -314,7 +342,8 @@ class Prepared:
             out=out_p, out2=out2_p, strad=_ptr(self.strad_t), n_strad=self.n_strad,
             fold_list=_ptr(self.fold_list), fold_count=_ptr(self.fold_cnt), fold_err=_ptr(self.fold_err),
             fold_cap=self.fold_cap, stages=stages, far_tiles=self.far_tiles,
-            fold_work=_ptr(self.fold_work), fold_work_items=self.fold_items, rule=self.rule)
+            fold_work=_ptr(self.fold_work), fold_work_items=self.fold_items, rule=self.rule,
+            near_tiles=_ptr(self.near_tiles_t))
 
     def chunks(self, max_pairs: int | None = None):
         """Observer-row ranges that together cover the job, each at most
@@ -345,7 +374,7 @@ class Prepared:
     def tune_rule(self):
         """Choose the mixed mode's far-field rule (near/far switch and Gauss
         orders by distance band) under a measured error bound: on the sample
-        rows the FP64 rule's far and near passes are the reference; every
+        rows the FP64 rule's passes (far, near, fold) are the reference; every
         candidate of RULE_CANDIDATES is run in the mixed mode on the same
         rows, its error max|mixed - FP64| / max|FP64| measured and its passes
         timed (HIP events, after one warm launch); the fastest candidate
@@ -366,7 +395,7 @@ class Prepared:
             return
         blocks = self._sample_blocks()
         row_elems = int(np.prod(self.final.shape[1:]))
-        stages = _lib.STAGE_PAIR | _lib.STAGE_Q
+        stages = 0        # every pass: the rule also splits the fold's points
         lib = dev.lib
 
         def run(flags, rule, tiles):

@@ -37,7 +37,7 @@ def test_sizes(lib):
     names = [f[0] for f in _lib.Launch._fields_]
     assert names.index("gq") == names.index("gl_w") + 1
     assert geometry.GQ_TOTAL == sum(n * n for n in range(1, 193))
-    assert names[-3:] == ["fold_work", "fold_work_items", "rule"]
+    assert names[-4:] == ["fold_work", "fold_work_items", "rule", "near_tiles"]
     assert C.sizeof(_lib.Rule) == 8 + 8 * _lib.RULE_BANDS + 4 * _lib.RULE_BANDS
     from mhs_amd import runtime
     for p in range(3):
