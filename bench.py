@@ -178,43 +178,80 @@ def shared_nodes_timing() -> list:
     return rows
 
 
-def c4_timing(steps: int = 2) -> dict:
-    """The three-fault network's continuous-layout interaction: about 6,000
-    triangles welded per fault into shared nodes, order 2, per-source eps,
-    receiver and source strike / dip, (nodes, nodes, 2, 2) FP64 left in HBM.
-    Wall ms per assembly in FP64 and in the mixed mode (one warm
-    assembly first), pair-evals/s counted as (receiver node, source element
-    node column) pairs, and the mixed result's relmax against FP64."""
+def _time_job(job, steps: int):
+    """Wall ms per assembly of a prepared job (one warm assembly first, its
+    rule and launch shape tuned), and the device result."""
     import torch
-    import mhs_amd as M
-    from mhs_amd import api, runtime, workloads
-    tris, groups, eps = workloads.c4_network()
-    sn = M.shared_nodes(tris, groups=groups)
-    mat = M.Material(30.0, 30.0)
-    out = {"n_tri": int(tris.shape[0]), "nodes": int(sn.n)}
-    res = {}
-    for mode in ("fp64", "mixed"):
-        with M.precision(mode):
-            job = api.plan(M.interaction_matrix, tris, mat, eps, order=2, layout="continuous", shared=sn,
-                           receiver=("strike", "dip"), source=("strike", "dip"))
-        prep = runtime.Prepared(job)
-        with torch.cuda.stream(prep.stream):
+    from mhs_amd import runtime
+    prep = runtime.Prepared(job)
+    with torch.cuda.stream(prep.stream):
+        prep.launch()
+        torch.cuda.synchronize()
+        prep.check_fold()
+        t0 = time.perf_counter()
+        for _ in range(steps):
             prep.launch()
-            torch.cuda.synchronize()
-            prep.check_fold()
-            t0 = time.perf_counter()
-            for _ in range(steps):
-                prep.launch()
-            torch.cuda.synchronize()
-        ms = 1e3 * (time.perf_counter() - t0) / steps
-        pairs = int(sn.n) * int(tris.shape[0]) * 6
+        torch.cuda.synchronize()
+    return 1e3 * (time.perf_counter() - t0) / steps, prep
+
+
+def _modes(plan, pairs: int, steps: int) -> dict:
+    """FP64 and mixed timings of one workload and the mixed result's relmax
+    against FP64 (max |mixed - FP64| / max |FP64| over every entry)."""
+    out, res = {}, {}
+    for mode in ("fp64", "mixed"):
+        ms, prep = _time_job(plan(mode), steps)
         out[mode] = {"ms": ms, "pair_evals_per_s": pairs / (ms / 1e3), "shape": list(prep.final.shape)}
         res[mode] = prep.final.clone()
         del prep
     d = (res["mixed"] - res["fp64"]).abs().max().item() / res["fp64"].abs().max().item()
     out["mixed_relmax_vs_fp64"] = d
+    out["mixed_over_fp64"] = out["fp64"]["ms"] / out["mixed"]["ms"]
     out["output_bytes"] = int(res["fp64"].numel() * 8)
     return out
+
+
+def c4_timing(steps: int = 2) -> dict:
+    """The three-fault network's continuous-layout interaction: about 6,000
+    triangles welded per fault into shared nodes, order 2, per-source eps,
+    the default receiver (strike, dip, normal) and source (strike, dip):
+    (nodes, nodes, 3, 2) FP64 left in HBM.  Wall ms per assembly in FP64 and
+    in the mixed mode (one warm assembly first), pair-evals/s counted as
+    (receiver node, source element node column) pairs, and the mixed
+    result's relmax against FP64."""
+    import mhs_amd as M
+    from mhs_amd import api, workloads
+    tris, groups, eps = workloads.c4_network()
+    sn = M.shared_nodes(tris, groups=groups)
+    mat = M.Material(30.0, 30.0)
+
+    def plan(mode):
+        with M.precision(mode):
+            return api.plan(M.interaction_matrix, tris, mat, eps, order=2, layout="continuous", shared=sn,
+                            source=("strike", "dip"))
+    out = {"n_tri": int(tris.shape[0]), "nodes": int(sn.n)}
+    out.update(_modes(plan, int(sn.n) * int(tris.shape[0]) * 6, steps))
+    return out
+
+
+def c2_timing(steps: int = 3) -> dict:
+    """The 2,000-triangle Gaussian-field fault: stress_matrix at 4,000
+    observers (4000, 3, 3, 2000, 3) and the default 3 x 3 interaction_matrix
+    (2000, 2000, 3, 3), FP64 and mixed, outputs left in HBM."""
+    import mhs_amd as M
+    from mhs_amd import api, workloads
+    obs, tris, eps = workloads.c2_stress()
+    mat = M.Material(30.0, 30.0)
+
+    def stress(mode):
+        with M.precision(mode):
+            return api.plan(M.stress_matrix, obs, tris, mat, eps)
+
+    def inter(mode):
+        with M.precision(mode):
+            return api.plan(M.interaction_matrix, tris, mat, eps)
+    return {"stress": _modes(stress, obs.shape[0] * tris.shape[0], steps),
+            "interaction_3x3": _modes(inter, tris.shape[0] * tris.shape[0], steps)}
 
 
 def _progress(msg: str) -> None:
@@ -301,6 +338,8 @@ def main() -> int:
                     help="time the device source-major copy of interaction-shaped matrices and exit")
     ap.add_argument("--c4", action="store_true",
                     help="time the three-fault network's continuous-layout order-2 interaction and exit")
+    ap.add_argument("--c2", action="store_true",
+                    help="time the Gaussian-field fault's stress and 3 x 3 interaction matrices and exit")
     ap.add_argument("--shared-nodes", action="store_true",
                     help="time the shared-node table, host against device, and exit")
     ap.add_argument("--counter-pass", action="store_true",
@@ -321,6 +360,9 @@ def main() -> int:
     if args.c4:
         print(json.dumps({"c4": c4_timing()}))
         return 0
+    if args.c2:
+        print(json.dumps({"c2": c2_timing()}))
+        return 0
     if args.source_major:
         print(json.dumps({"source_major": source_major_timing()}))
         return 0
@@ -340,7 +382,7 @@ def main() -> int:
     relerr = float((d.max() / b.abs().max()).item())
     # per receiver row: the row's largest error over the row's largest entry
     row_relerr = float((d.amax(dim=(1, 2, 3)) / b.abs().amax(dim=(1, 2, 3))).max().item())
-    rule_key = prep.tune_key()[:2] + (True,) + prep.tune_key()[3:]      # the mixed run's tuner entry
+    rule_key = ("interaction",) + prep.tune_key()[:2] + (True,) + prep.tune_key()[3:]   # the mixed run's entry
     del prep, ref, a, b, d
 
     n = 10_000

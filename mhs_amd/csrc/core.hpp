@@ -643,8 +643,33 @@ MHS_INLINE double centroid_distance(const double* c, const double o[3], double e
   return sqrt(fma(d0, d0, fma(d1, d1, fma(d2, d2, eps * eps))));
 }
 
+// Quadrature rule of the mixed mode.  A pair is near (closed form) when
+// sqrt(|x - c|^2 + eps^2) <= near L; otherwise each far-field family
+// (direct on the source triangle; image R and Q on the mirrored one, from
+// its centroid) takes the Gauss order of the first band with that distance
+// <= hi[b] L, plus the nodal order P.  The all-FP64 mode keeps the
+// reference's rule (near 10 L, 12 x 12 to 40 L, 8 x 8 beyond, the Q budget
+// law) and never reads this.
+#ifndef MHS_RULE_BANDS
+#define MHS_RULE_BANDS 8
+#endif
+struct Rule {
+  double near;
+  double hi[MHS_RULE_BANDS];
+  int n[MHS_RULE_BANDS];
+};
+
+// Gauss order of a family at distance D from a triangle of size L.
+MHS_INLINE int rule_order(const Rule& r, double D, double L) {
+#pragma unroll
+  for (int b = 0; b < MHS_RULE_BANDS - 1; ++b)
+    if (D <= r.hi[b] * L) return r.n[b];
+  return r.n[MHS_RULE_BANDS - 1];
+}
+
 // Weighted tables of node k by the hybrid switch (moments.py:551-601);
-// returns the plane height z.
+// returns the plane height z.  With a mixed rule: its near switch and far
+// orders instead of 10 L and 12^2 / 8^2.
 // Coordinates of o in the frame f: returns the normal offset z; X1, X2 in
 // the plane, h2 = z^2 + eps^2.
 MHS_INLINE double frame_local(const Frame& f, const double o[3], double eps, double& X1, double& X2,
@@ -661,19 +686,20 @@ template <class S, int P>
 MHS_INLINE double weighted_tables(const Frame& f, const double* __restrict__ v, int k,
                                   const double o[3], double eps,
                                   const double* __restrict__ gl_x, const double* __restrict__ gl_w,
-                                  double W[6][21], double& X1o, double& X2o) {
+                                  double W[6][21], double& X1o, double& X2o, double near_r = 10.0,
+                                  const Rule* rule = nullptr) {
   double X1, X2, h2;
   const double z = frame_local(f, o, eps, X1, X2, h2);
   X1o = X1; X2o = X2;
   const double Dn = centroid_distance(f.c, o, eps);
-  if (Dn <= 10.0 * f.L) {
+  if (Dn <= near_r * f.L) {
     double M[6][36];
     raw_moments_near<S>(f, v, X1, X2, h2, M);
     double c[6];
     shape_coeffs<P>(f, k, X1, X2, c);
     weight_from_raw<S, P>(M, c, W);
   } else {
-    const int ng = (Dn > 40.0 * f.L) ? 8 : 12;
+    const int ng = rule ? rule_order(*rule, Dn, f.L) + P : ((Dn > 40.0 * f.L) ? 8 : 12);
     const int off = ng * (ng - 1) / 2;
     weighted_far<S, P>(f, k, X1, X2, h2, ng, gl_x + off, gl_w + off, W);
   }

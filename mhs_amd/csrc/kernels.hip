@@ -216,7 +216,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   const double* src = pr.src + s * MHS_SRC_STRIDE;
   const double ob[3] = {pr.obs[3 * o], pr.obs[3 * o + 1], pr.obs[3 * o + 2]};
   if (in_fold_band(src, ob, pr.flags)) return;        // the fold replaces R + Q there
-  const int nq = q_order_fast<P>(src, ob);
+  int nq;
+  if (pr.mixed_rule) {
+    // the mixed rule's order for a far pair, the budget law for a near one
+    const double Di = centroid_distance(src + S_IC, ob, src[S_EPS]);
+    nq = Di <= pr.near_r * src[S_IL] ? q_order_fast<P>(src, ob) : rule_order(pr.rule, Di, src[S_IL]) + P;
+  } else {
+    nq = q_order_fast<P>(src, ob);
+  }
   const double* n = src + S_N;
   const double* v = src + S_V;
   const double eps2 = src[S_EPS] * src[S_EPS];
@@ -451,7 +458,7 @@ k_far(Params pr) {
   // receiver half E_a (strain-space traction functional, frame row a)
   if (tid < n_a * 9) {
     const int a = tid / 9, i = (tid % 9) / 3, p = tid % 3;
-    const double* rb = pr.rb + ((int64_t)o * n_a + a) * 3;
+    const double* rb = pr.rb + ((int64_t)o * pr.na_tot + a) * 3;
     const double rn = rb[0] * nr[0] + rb[1] * nr[1] + rb[2] * nr[2];
     E_s[a][i * 3 + p] = pr.mu * (rb[i] * nr[p] + rb[p] * nr[i]) + (i == p ? pr.lam * rn : 0.0);
   }
@@ -507,7 +514,7 @@ k_far(Params pr) {
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
         if (b < n_b) {
-          const double* sb = pr.sb + (s * n_b + b) * 3;
+          const double* sb = pr.sb + (s * pr.nb_tot + b) * 3;
           const double sn = sb[0] * n[0] + sb[1] * n[1] + sb[2] * n[2];
           F[b][0] = 2.0 * pr.mu * sb[0] * n[0] + pr.lam * sn;
           F[b][1] = 2.0 * pr.mu * sb[1] * n[1] + pr.lam * sn;
@@ -561,7 +568,7 @@ k_far(Params pr) {
           for (int t = 0; t < 9; ++t) EF = fma(Ea[t], Fm[t], EF);
           const double trE = Ea[0] + Ea[4] + Ea[8], trF = Fm[0] + Fm[4] + Fm[8];
           const double eps = src[S_EPS], e2 = eps * eps;
-          const double* rb = pr.rb + ((int64_t)o * n_a + a) * 3;
+          const double* rb = pr.rb + ((int64_t)o * pr.na_tot + a) * 3;
           double rFn = 0.0;
 #pragma unroll
           for (int i = 0; i < 3; ++i)
@@ -626,8 +633,13 @@ k_far(Params pr) {
       const int64_t s0 = ((int64_t)blockIdx.x * pr.far_tiles + tile) * PPB;
       const int64_t oo = o - pr.out_lo;
       const int64_t n_here = (pr.n_src - s0 < PPB ? pr.n_src - s0 : PPB) * K * NOUT;
-      double* dst = pr.out + (oo * pr.n_cols + s0 * K) * NOUT;
-      for (int64_t i = tid; i < n_here; i += 256) dst[i] = res_s[i];
+      const int os = pr.na_tot * pr.nb_tot;   // entries per (row, column); more than NOUT in a group launch
+      double* dst = pr.out + (oo * pr.n_cols + s0 * K) * os;
+      if (os == NOUT) {
+        for (int64_t i = tid; i < n_here; i += 256) dst[i] = res_s[i];
+      } else {
+        for (int64_t i = tid; i < n_here; i += 256) dst[(i / NOUT) * os + i % NOUT] = res_s[i];
+      }
     }
   }
 }
@@ -786,7 +798,7 @@ k_far_mixed(Params pr) {
   __shared__ __align__(16) float BQ_s[2][MHS_QM * 8];
   __shared__ __align__(16) float BR_s[2][MHS_RM * 8];
   __shared__ double E_s[2][9];
-  __shared__ double res_s[KN > 1 ? 256 * KN : 1];
+  __shared__ double res_s[256 * KN];
   const int tid = threadIdx.x;
   const int n_a = pr.n_a, n_b = pr.n_b;
   const int64_t o = pr.obs_lo + blockIdx.y;
@@ -794,7 +806,7 @@ k_far_mixed(Params pr) {
   const double* nr = pr.nrm + 3 * o;
   if (tid < n_a * 9) {
     const int a = tid / 9, i = (tid % 9) / 3, p = tid % 3;
-    const double* rb = pr.rb + ((int64_t)o * n_a + a) * 3;
+    const double* rb = pr.rb + ((int64_t)o * pr.na_tot + a) * 3;
     const double rn = rb[0] * nr[0] + rb[1] * nr[1] + rb[2] * nr[2];
     E_s[a][i * 3 + p] = pr.mu * (rb[i] * nr[p] + rb[p] * nr[i]) + (i == p ? pr.lam * rn : 0.0);
   }
@@ -839,7 +851,7 @@ k_far_mixed(Params pr) {
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
         if (b < n_b) {
-          const double* sb = pr.sb + (s * n_b + b) * 3;
+          const double* sb = pr.sb + (s * pr.nb_tot + b) * 3;
           half_tensor(sb, n, pr.mu, pr.lam, F[b]);
         } else {
 #pragma unroll
@@ -878,7 +890,7 @@ k_far_mixed(Params pr) {
           for (int q = 0; q < 9; ++q) EF = fma(Ea[q], Fm[q], EF);
           const double trE = Ea[0] + Ea[4] + Ea[8], trF = Fm[0] + Fm[4] + Fm[8];
           const double e2 = eps * eps;
-          const double* rb = pr.rb + ((int64_t)o * n_a + (oa[t] < n_a ? oa[t] : 0)) * 3;
+          const double* rb = pr.rb + ((int64_t)o * pr.na_tot + (oa[t] < n_a ? oa[t] : 0)) * 3;
           double rFn = 0.0;
 #pragma unroll
           for (int i = 0; i < 3; ++i)
@@ -913,7 +925,8 @@ k_far_mixed(Params pr) {
       }
     }
     const int64_t oo = o - pr.out_lo;
-    if (KN == 1) {
+    const int os = pr.na_tot * pr.nb_tot;   // entries per (row, column); more than NOUT in a group launch
+    if (KN == 1 && os == 1) {
       if (s < pr.n_src) pr.out[oo * pr.n_cols + s] = acc[0][0];
     } else {
 #pragma unroll
@@ -924,8 +937,12 @@ k_far_mixed(Params pr) {
       // the tile's sources are 256 K consecutive columns: one contiguous run of the row
       const int64_t s0 = ((int64_t)blockIdx.x * pr.far_tiles + tile) * 256;
       const int64_t n_here = (pr.n_src - s0 < 256 ? pr.n_src - s0 : 256) * KN;
-      double* dst = pr.out + (oo * pr.n_cols + s0 * K) * NOUT;
-      for (int64_t i = tid; i < n_here; i += 256) dst[i] = res_s[i];
+      double* dst = pr.out + (oo * pr.n_cols + s0 * K) * os;
+      if (os == NOUT) {
+        for (int64_t i = tid; i < n_here; i += 256) dst[i] = res_s[i];
+      } else {
+        for (int64_t i = tid; i < n_here; i += 256) dst[(i / NOUT) * os + i % NOUT] = res_s[i];
+      }
       __syncthreads();
     }
   }
@@ -1409,11 +1426,29 @@ static Rule default_rule() {
   return r;
 }
 
+// Whether an interaction launch runs the contracted passes (pointwise far
+// field per output group of one or two components, each point once for all
+// K node columns) or the tensor passes (moment tables lifted to all 27
+// stress components, per node column).  One or two components: contracted.
+// More, in the mixed mode: contracted while its output groups (a receiver
+// row with up to two source rows each) are no more than the node columns K
+// -- measured: the three-fault network at order 2 (3 x 2 components, K = 6)
+// 1.03 s -> 0.55 s, the Gaussian-field fault at order 0 (3 x 3, K = 1)
+// 10 ms -> 16 ms.  The FP64 mode keeps the tensor passes beyond two
+// components (faster there: 2.2 s against 2.8 s on the network).
+static bool use_contracted(int kind, bool has_sb, int n_a, int n_b, int order, int flags) {
+  if (kind != K_INTERACTION || !has_sb) return false;
+  if (n_a * n_b <= 2) return true;
+  const int K = (order + 1) * (order + 2) / 2;
+  const int groups = n_a * ((n_b + 1) / 2);
+  return (flags & F_MIXED) && groups <= K;
+}
+
 // The near/far switch and the far-field rule of a launch: the reference's
 // 10 L, or with MHS_MIXED on the contracted interaction the descriptor's
 // rule (checked) or the default.
 static int fill_rule(const mhs_launch* L, Params& pr) {
-  const bool mixed_rule = L->kind == K_INTERACTION && L->sb && L->n_a * L->n_b <= 2 && (L->flags & F_MIXED);
+  const bool mixed_rule = (L->flags & F_MIXED) != 0;
   pr.near_r = 10.0;
   pr.rule = default_rule();
   pr.mixed_rule = mixed_rule ? 1 : 0;
@@ -1450,7 +1485,7 @@ static int launch_all(const mhs_launch* L, const Params& pr, hipStream_t st) {
   const bool stress = pr.kind != K_DISP;
   const int stages = L->stages ? L->stages : (MHS_STAGE_PAIR | MHS_STAGE_Q | MHS_STAGE_FOLD);
   const int nout = pr.n_a * pr.n_b;
-  const bool contracted = pr.kind == K_INTERACTION && pr.sb && nout <= 2;
+  const bool contracted = use_contracted(pr.kind, pr.sb != nullptr, pr.n_a, pr.n_b, P, pr.flags);
   const bool mixed = pr.flags & F_MIXED;
   // points per lane (two outputs: half as many)
   constexpr int N64 = MHS_FAR_NPT64;
@@ -1462,24 +1497,40 @@ static int launch_all(const mhs_launch* L, const Params& pr, hipStream_t st) {
     const dim3 grid((unsigned)nblk, (unsigned)rows);
     const dim3 tiles((unsigned)((pr.n_cols + MHS_ROW_TILE - 1) / MHS_ROW_TILE), (unsigned)rows);   // column tiles
     if (contracted) {
-      // far pass (writes every entry), then the near pass (adds)
-      if (stages & MHS_STAGE_Q) {
-        auto fg = [&](int lpp) {   // sources per block = 256 / lanes per pair, per tile
-          const int64_t per = (int64_t)(256 / lpp) * pr.far_tiles;
-          return dim3((unsigned)((pr.n_src + per - 1) / per), (unsigned)rows);
-        };
-        constexpr int L64 = MHS_FAR_LPP64, L64b = MHS_FAR_LPP64B;
-        // the mixed mode: a lane per pair at the mixed rule's orders
-        if (nout == 1 && mixed) hipLaunchKernelGGL((k_far_mixed<P, 1>), fg(1), block, 0, st, pc);
-        else if (nout == 1) hipLaunchKernelGGL((k_far<double, P, 1, N64, L64>), fg(L64), block, 0, st, pc);
-        else if (mixed) hipLaunchKernelGGL((k_far_mixed<P, 2>), fg(1), block, 0, st, pc);
-        else hipLaunchKernelGGL((k_far<double, P, 2, N64b, L64b>), fg(L64b), block, 0, st, pc);
-        if (check(hipGetLastError(), "k_far launch")) return -1;
-      }
-      if (stages & MHS_STAGE_PAIR) {
-        hipLaunchKernelGGL((k_near<P>), tiles, block, 0, st, pc);
-        if (check(hipGetLastError(), "k_near launch")) return -1;
-      }
+      // output groups: the whole launch when it has one or two outputs, else
+      // one receiver frame row a with source rows b0, b0 + 1 per group (the
+      // pointers offset to the group, the strides those of the whole launch)
+      const bool grouped = nout > 2;
+      for (int a = 0; a < (grouped ? pr.n_a : 1); ++a)
+        for (int b0 = 0; b0 < (grouped ? pr.n_b : 1); b0 += 2) {
+          Params pg = pc;
+          if (grouped) {
+            pg.rb = pr.rb + a * 3;
+            pg.sb = pr.sb + b0 * 3;
+            pg.out = pr.out + (a * pr.n_b + b0);
+            pg.n_a = 1;
+            pg.n_b = pr.n_b - b0 < 2 ? pr.n_b - b0 : 2;
+          }
+          const int gout = pg.n_a * pg.n_b;
+          // far pass (writes every entry of the group), then the near pass (adds)
+          if (stages & MHS_STAGE_Q) {
+            auto fg = [&](int lpp) {   // sources per block = 256 / lanes per pair, per tile
+              const int64_t per = (int64_t)(256 / lpp) * pr.far_tiles;
+              return dim3((unsigned)((pr.n_src + per - 1) / per), (unsigned)rows);
+            };
+            constexpr int L64 = MHS_FAR_LPP64, L64b = MHS_FAR_LPP64B;
+            // the mixed mode: a lane per pair at the mixed rule's orders
+            if (gout == 1 && mixed) hipLaunchKernelGGL((k_far_mixed<P, 1>), fg(1), block, 0, st, pg);
+            else if (gout == 1) hipLaunchKernelGGL((k_far<double, P, 1, N64, L64>), fg(L64), block, 0, st, pg);
+            else if (mixed) hipLaunchKernelGGL((k_far_mixed<P, 2>), fg(1), block, 0, st, pg);
+            else hipLaunchKernelGGL((k_far<double, P, 2, N64b, L64b>), fg(L64b), block, 0, st, pg);
+            if (check(hipGetLastError(), "k_far launch")) return -1;
+          }
+          if (stages & MHS_STAGE_PAIR) {
+            hipLaunchKernelGGL((k_near<P>), tiles, block, 0, st, pg);
+            if (check(hipGetLastError(), "k_near launch")) return -1;
+          }
+        }
       continue;
     }
     if (stages & MHS_STAGE_PAIR) {
@@ -1517,7 +1568,10 @@ static int launch_all(const mhs_launch* L, const Params& pr, hipStream_t st) {
         hipLaunchKernelGGL((k_fold<P, true, false>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items);
     } else {
       hipLaunchKernelGGL((k_fold<P, false, true>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items);
-      hipLaunchKernelGGL((k_fold<P, false, false>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items);
+      if (pr.mixed_rule)
+        hipLaunchKernelGGL((k_fold<P, false, false, true>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items);
+      else
+        hipLaunchKernelGGL((k_fold<P, false, false>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items);
     }
     if (check(hipGetLastError(), "k_fold launch")) return -1;
   }
@@ -1625,7 +1679,9 @@ int mhs_assemble(const mhs_launch* L, void* stream) {
     pr.k5c = 3.0 * c34 - 9.0;
   }
   if (L->far_tiles < 0 || L->far_tiles > 4096) return fail("mhs_assemble: far_tiles out of range");
-  const bool contracted = L->kind == K_INTERACTION && L->sb && L->n_a * L->n_b <= 2;
+  const bool contracted = use_contracted(L->kind, L->sb != nullptr, L->n_a, L->n_b, L->order, L->flags);
+  pr.na_tot = L->n_a;
+  pr.nb_tot = L->n_b;
   pr.far_tiles = L->far_tiles ? L->far_tiles : ((contracted && (L->flags & F_MIXED)) ? MHS_FARM_CT : MHS_QC_CT);
   if (fill_rule(L, pr)) return -1;
   hipStream_t st = (hipStream_t)stream;

@@ -9,30 +9,6 @@
 enum Kind { K_DISP = 0, K_STRESS = 1, K_EIGEN = 2, K_TRACTION = 3, K_INTERACTION = 4 };
 enum Flag { F_FOLD = 1, F_SUBTRACT = 2, F_PARTS = 4, F_MIXED = 8 };
 
-// Quadrature rule of the mixed mode's contracted far pass.  A pair is near
-// (closed form) when sqrt(|x - c|^2 + eps^2) <= near L; otherwise its
-// direct, image R and image Q families each take the Gauss order of the
-// first band with that distance <= hi[b] L (direct: the source's centroid;
-// R and Q: the mirrored centroid), plus the nodal order P.  The all-FP64
-// mode keeps the reference's rule (near 10, 12 x 12 to 40, 8 x 8 beyond,
-// the Q budget law) and never reads this.
-#ifndef MHS_RULE_BANDS
-#define MHS_RULE_BANDS 8
-#endif
-struct Rule {
-  double near;
-  double hi[MHS_RULE_BANDS];
-  int n[MHS_RULE_BANDS];
-};
-
-// Gauss order of a family at distance D from a triangle of size L.
-MHS_INLINE int rule_order(const Rule& r, double D, double L) {
-#pragma unroll
-  for (int b = 0; b < MHS_RULE_BANDS - 1; ++b)
-    if (D <= r.hi[b] * L) return r.n[b];
-  return r.n[MHS_RULE_BANDS - 1];
-}
-
 struct Params {
   const double* obs;     // n_obs x 3
   const double* nrm;     // n_obs x 3 (traction, interaction)
@@ -49,6 +25,10 @@ struct Params {
   int64_t obs_lo;        // first observer row of this launch
   int64_t out_lo;        // observer row held by the first row of out / out2
   int kind, flags, n_a, n_b;
+  // the rows of rb / sb and the entries of out per (row, column): n_a, n_b
+  // and n_a n_b, except in an output group of the contracted interaction
+  // (n_a = 1, n_b <= 2 of the whole launch's na_tot x nb_tot)
+  int na_tot, nb_tot;
   int far_tiles;         // column tiles per block of the contracted far pass
   double mu, lam, nu;
   // constants of the contracted direct series, computed once on the host so
@@ -66,6 +46,9 @@ struct Params {
   // source (1/0); or null
   const double* near_tiles;
 };
+
+// The launch's mixed rule, or null for the reference's rule.
+MHS_INLINE const Rule* rule_of(const Params& pr) { return pr.mixed_rule ? &pr.rule : nullptr; }
 
 // Fold classification of one (receiver, source) pair (fold.py:84-118): the
 // source straddles and the receiver is shallower than 10 eps and within
@@ -103,7 +86,7 @@ MHS_INLINE void pair_stress(const Params& pr, const double* __restrict__ src, in
   {
     Frame f;
     load_frame(src, false, f);
-    const double z = weighted_tables<SpecH<P>, P>(f, src + S_V, k, o, eps, pr.gl_x, pr.gl_w, W, X1, X2);
+    const double z = weighted_tables<SpecH<P>, P>(f, src + S_V, k, o, eps, pr.gl_x, pr.gl_w, W, X1, X2, pr.near_r, rule_of(pr));
     phi_d = direct_E(W, eps);
     if (want_H) direct_H(f, W, z, eps, mt, H);
   }
@@ -114,7 +97,7 @@ MHS_INLINE void pair_stress(const Params& pr, const double* __restrict__ src, in
   reflected_vertices(src, vi);
   if (band) {
     // mirror element, full space, slip columns signed (-1, -1, +1) (fold.py:370-380)
-    const double z = weighted_tables<SpecH<P>, P>(g, vi, k, o, eps, pr.gl_x, pr.gl_w, W, X1, X2);
+    const double z = weighted_tables<SpecH<P>, P>(g, vi, k, o, eps, pr.gl_x, pr.gl_w, W, X1, X2, pr.near_r, rule_of(pr));
     phi_m = direct_E(W, eps);
     if (want_H) {
       double Hm[27];
@@ -124,7 +107,7 @@ MHS_INLINE void pair_stress(const Params& pr, const double* __restrict__ src, in
     }
   } else if (want_H) {
     // image R-family in closed form on the reflected triangle (image.py:435-497)
-    const double zp = weighted_tables<SpecI<P>, P>(g, vi, k, o, eps, pr.gl_x, pr.gl_w, W, X1, X2);
+    const double zp = weighted_tables<SpecI<P>, P>(g, vi, k, o, eps, pr.gl_x, pr.gl_w, W, X1, X2, pr.near_r, rule_of(pr));
     const double al[3] = {-g.e1[0], -g.e1[1], -g.e1[2]};
     const double be[3] = {-g.e2[0], -g.e2[1], -g.e2[2]};
     const double ga[3] = {zp * g.n[0], zp * g.n[1], zp * g.n[2]};
@@ -150,7 +133,7 @@ MHS_INLINE void pair_disp(const Params& pr, const double* __restrict__ src, int 
   {
     Frame f;
     load_frame(src, false, f);
-    const double z = weighted_tables<SpecU<P>, P>(f, src + S_V, k, o, eps, pr.gl_x, pr.gl_w, W, X1, X2);
+    const double z = weighted_tables<SpecU<P>, P>(f, src + S_V, k, o, eps, pr.gl_x, pr.gl_w, W, X1, X2, pr.near_r, rule_of(pr));
     direct_U(f, W, z, eps, mt, U);
   }
   Frame g;
@@ -158,13 +141,13 @@ MHS_INLINE void pair_disp(const Params& pr, const double* __restrict__ src, int 
   double vi[9];
   reflected_vertices(src, vi);
   if (band) {
-    const double z = weighted_tables<SpecU<P>, P>(g, vi, k, o, eps, pr.gl_x, pr.gl_w, W, X1, X2);
+    const double z = weighted_tables<SpecU<P>, P>(g, vi, k, o, eps, pr.gl_x, pr.gl_w, W, X1, X2, pr.near_r, rule_of(pr));
     double Um[9];
     direct_U(g, W, z, eps, mt, Um);
 #pragma unroll
     for (int t = 0; t < 9; ++t) U[t] += ((t % 3) == 2 ? Um[t] : -Um[t]);
   } else {
-    const double zp = weighted_tables<SpecI<P>, P>(g, vi, k, o, eps, pr.gl_x, pr.gl_w, W, X1, X2);
+    const double zp = weighted_tables<SpecI<P>, P>(g, vi, k, o, eps, pr.gl_x, pr.gl_w, W, X1, X2, pr.near_r, rule_of(pr));
     const double al[3] = {-g.e1[0], -g.e1[1], -g.e1[2]};
     const double be[3] = {-g.e2[0], -g.e2[1], -g.e2[2]};
     const double ga[3] = {zp * g.n[0], zp * g.n[1], zp * g.n[2]};
@@ -397,14 +380,14 @@ MHS_INLINE void pair_near_item(const Params& pr, int64_t o, int64_t col) {
   // receiver halves E_a, source halves F_b (element) and G_b (folded mirror:
   // slip (-sx, -sy, sz) on the mirror normal, fold.py:370-380)
   double Ea[2][6], Fb[2][6], Gb[2][6], rF[2][2], rG[2][2];
-  for (int a = 0; a < n_a; ++a) half_tensor(pr.rb + ((int64_t)o * n_a + a) * 3, nr, pr.mu, pr.lam, Ea[a]);
+  for (int a = 0; a < n_a; ++a) half_tensor(pr.rb + ((int64_t)o * pr.na_tot + a) * 3, nr, pr.mu, pr.lam, Ea[a]);
   for (int b = 0; b < n_b; ++b) {
-    const double* sb = pr.sb + (s * n_b + b) * 3;
+    const double* sb = pr.sb + (s * pr.nb_tot + b) * 3;
     half_tensor(sb, src + S_N, pr.mu, pr.lam, Fb[b]);
     const double sm[3] = {-sb[0], -sb[1], sb[2]};
     half_tensor(sm, src + S_IN, pr.mu, pr.lam, Gb[b]);
     for (int a = 0; a < n_a; ++a) {
-      const double* rb = pr.rb + ((int64_t)o * n_a + a) * 3;
+      const double* rb = pr.rb + ((int64_t)o * pr.na_tot + a) * 3;
       double f = 0.0, g = 0.0;
       const int ix[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
       for (int i = 0; i < 3; ++i)
@@ -422,7 +405,7 @@ MHS_INLINE void pair_near_item(const Params& pr, int64_t o, int64_t col) {
   if (dnear) {
     Frame f;
     load_frame(src, false, f);
-    const double z = weighted_tables<SpecH<P>, P>(f, src + S_V, k, ob, eps, pr.gl_x, pr.gl_w, W, X1, X2);
+    const double z = weighted_tables<SpecH<P>, P>(f, src + S_V, k, ob, eps, pr.gl_x, pr.gl_w, W, X1, X2, pr.near_r, rule_of(pr));
     for (int a = 0; a < n_a; ++a)
       for (int b = 0; b < n_b; ++b)
         acc[a * n_b + b] += direct_contract(f, W, z, eps, mt, Ea[a], Fb[b], sub, rF[a][b]);
@@ -433,12 +416,12 @@ MHS_INLINE void pair_near_item(const Params& pr, int64_t o, int64_t col) {
     double vi[9];
     reflected_vertices(src, vi);
     if (band) {
-      const double z = weighted_tables<SpecH<P>, P>(g, vi, k, ob, eps, pr.gl_x, pr.gl_w, W, X1, X2);
+      const double z = weighted_tables<SpecH<P>, P>(g, vi, k, ob, eps, pr.gl_x, pr.gl_w, W, X1, X2, pr.near_r, rule_of(pr));
       for (int a = 0; a < n_a; ++a)
         for (int b = 0; b < n_b; ++b)
           acc[a * n_b + b] += direct_contract(g, W, z, eps, mt, Ea[a], Gb[b], sub, rG[a][b]);
     } else {
-      const double zp = weighted_tables<SpecI<P>, P>(g, vi, k, ob, eps, pr.gl_x, pr.gl_w, W, X1, X2);
+      const double zp = weighted_tables<SpecI<P>, P>(g, vi, k, ob, eps, pr.gl_x, pr.gl_w, W, X1, X2, pr.near_r, rule_of(pr));
       const double al[3] = {-g.e1[0], -g.e1[1], -g.e1[2]};
       const double be[3] = {-g.e2[0], -g.e2[1], -g.e2[2]};
       const double ga[3] = {zp * g.n[0], zp * g.n[1], zp * g.n[2]};
@@ -461,6 +444,7 @@ MHS_INLINE void pair_near_item(const Params& pr, int64_t o, int64_t col) {
     }
   }
   const int64_t oo = o - pr.out_lo;
-  for (int t = 0; t < n_a * n_b; ++t) pr.out[(oo * pr.n_cols + col) * (n_a * n_b) + t] += acc[t];
+  const int os = pr.na_tot * pr.nb_tot;     // entries per (row, column); more in a group launch
+  for (int t = 0; t < n_a * n_b; ++t) pr.out[(oo * pr.n_cols + col) * os + t] += acc[t];
 }
 

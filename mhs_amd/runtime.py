@@ -227,6 +227,20 @@ def _ptr(t):
     return None if t is None else C.c_void_p(t.data_ptr())
 
 
+def use_contracted(kind: str, has_sb: bool, n_a: int, n_b: int, order: int, mixed: bool) -> bool:
+    """The library's choice between the contracted passes and the tensor
+    passes for an interaction launch (csrc/kernels.hip use_contracted): one
+    or two components always; more in the mixed mode while the output groups
+    (a receiver row with up to two source rows) are no more than the node
+    columns per source."""
+    if kind != "interaction" or not has_sb:
+        return False
+    if n_a * n_b <= 2:
+        return True
+    K = (order + 1) * (order + 2) // 2
+    return bool(mixed) and n_a * ((n_b + 1) // 2) <= K
+
+
 def near_tile_bounds(recs: np.ndarray, K: int) -> np.ndarray:
     """Per 256 element columns (n_cols = n_src K): the bounding box of the
     covered sources' centroids and mirrored centroids (lo xyz, hi xyz), their
@@ -284,7 +298,10 @@ class Prepared:
             self.src_t = dev.put(recs)
             self.nrm_t = dev.put(job.nrm)
             self.rb_t = dev.put(job.rb)
-            self.sb_t = None if cont else dev.put(job.sb)
+            # the continuous layout's element columns take Cartesian slip
+            # (resolved on the node frames by the gather): unit source rows
+            self.sb_t = (dev.put(np.broadcast_to(np.eye(3), (job.n_src, 3, 3))) if cont and job.kind == "interaction"
+                         else None if cont else dev.put(job.sb))
             self.strad_t = dev.put(strad, np.int64) if strad.size else None
             self.coef_t = dev.coef(job.mu, job.lam)
             self.final = torch.empty(job.out_shape(), dtype=torch.float64, device=dev.dev)
@@ -318,10 +335,11 @@ class Prepared:
         self.flags = ((_lib.F_FOLD if job.fold else 0) | (_lib.F_SUBTRACT if job.subtract else 0)
                       | (_lib.F_PARTS if job.parts else 0) | (_lib.F_MIXED if job.mixed else 0))
         self.n_b_launch = 3 if (cont and job.kind == "interaction") else job.n_b
-        # the contracted far pass (interaction, one or two components) has a
-        # tunable launch shape: column tiles per block
-        self.contracted = (job.kind == "interaction" and not cont and job.sb is not None
-                           and job.n_a * job.n_b <= 2)
+        # the interaction runs the contracted passes (pointwise far field,
+        # closed-form near field, in output groups of one or two components),
+        # whose far pass has a tunable launch shape: column tiles per block
+        self.contracted = use_contracted(job.kind, self.sb_t is not None, job.n_a, self.n_b_launch,
+                                         job.order, job.mixed)
         self.near_tiles_t = None
         if self.contracted:
             with torch.cuda.stream(self.stream):
@@ -365,11 +383,19 @@ class Prepared:
         mixes), or all rows."""
         n = self.job.n_obs
         rows = int(min(n, max(1, AUTOTUNE_PAIRS // max(self.n_cols, 1))))
-        if rows >= n:
+        if rows >= n and n <= self.rows:
             return [(0, n)]
-        per = max(1, rows // 3)
+        per = max(1, min(rows // 3, self.rows))     # a block fits the continuous layout's scratch
         return sorted({(max(0, min(n - per, int(f * n) - per // 2)),
                         max(0, min(n - per, int(f * n) - per // 2)) + per) for f in (1 / 6, 1 / 2, 5 / 6)})
+
+    def fix_rule(self):
+        """Use the rule fixed in ``mixed_rule``, else the device's tuned rule
+        for this job's class if there is one, else the library default --
+        without tuning (for callers that must not synchronise)."""
+        key = (self.job.kind,) + self.tune_key()
+        self.rule = rule_struct(mixed_rule if mixed_rule is not None else self.dev.rules.get(key))
+        self.rule_tuned = True
 
     def tune_rule(self):
         """Choose the mixed mode's far-field rule (near/far switch and Gauss
@@ -379,19 +405,25 @@ class Prepared:
         rows, its error max|mixed - FP64| / max|FP64| measured and its passes
         timed (HIP events, after one warm launch); the fastest candidate
         within RULE_ERROR_BOUND wins (none within: the most conservative).
-        Cached per (order, outputs, size class) on the device with every
-        candidate's error and time.  A rule fixed in ``mixed_rule`` is used
-        untuned.  Synchronises the stream."""
-        if not (self.contracted and self.job.mixed) or self.rule_tuned:
+        Cached per (form, order, outputs, size class) on the device with
+        every candidate's error and time.  A rule fixed in ``mixed_rule`` is
+        used untuned; so is the library's default for the continuous layout
+        and for jobs smaller than the sample (AUTOTUNE_PAIRS pairs), which a
+        tuned rule of their class replaces once one exists.  Synchronises
+        the stream."""
+        if not self.job.mixed or self.rule_tuned:
             return
         self.rule_tuned = True
         dev, torch = self.dev, self.dev.torch
         if mixed_rule is not None or not autotune:
             self.rule = rule_struct(mixed_rule)
             return
-        key = self.tune_key()
+        key = (self.job.kind,) + self.tune_key()
         if key in dev.rules:
             self.rule = rule_struct(dev.rules[key])
+            return
+        if self.cont or self.job.n_obs * self.n_cols < AUTOTUNE_PAIRS:
+            self.rule = rule_struct(None)
             return
         blocks = self._sample_blocks()
         row_elems = int(np.prod(self.final.shape[1:]))
@@ -400,8 +432,10 @@ class Prepared:
 
         def run(flags, rule, tiles):
             for lo, hi in blocks:
-                L = self._launch_struct(lo, hi, C.c_void_p(self.final.data_ptr() + 8 * lo * row_elems),
-                                        None, stages)
+                off = 8 * lo * row_elems
+                L = self._launch_struct(lo, hi, C.c_void_p(self.final.data_ptr() + off),
+                                        C.c_void_p(self.final2.data_ptr() + off) if self.final2 is not None
+                                        else None, stages)
                 L.flags, L.rule, L.far_tiles = flags, rule, tiles
                 _lib.check(lib.mhs_assemble(C.byref(L), self.handle), lib)
 
@@ -480,7 +514,7 @@ class Prepared:
         job, lib = self.job, self.dev.lib
         if self.contracted and not self.far_tiles:
             self.tune()
-        if self.contracted and job.mixed and not self.rule_tuned:
+        if job.mixed and not self.rule_tuned:
             self.tune_rule()
         if self.cont:
             if hi - lo > self.rows:
@@ -524,7 +558,7 @@ class Prepared:
         if self.contracted and not self.far_tiles:
             self.tune()
             torch.cuda.synchronize()
-        if self.contracted and self.job.mixed and not self.rule_tuned:
+        if self.job.mixed and not self.rule_tuned:
             self.tune_rule()
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g, stream=self.stream):

@@ -99,7 +99,9 @@ class GpuBackend:
             self.torch.cuda.empty_cache()
 
     def prepare(self, job: runtime.Job, slot: int):
-        return runtime.Prepared(job, stream=self.streams[slot], dev=self.dev)
+        prep = runtime.Prepared(job, stream=self.streams[slot], dev=self.dev)
+        prep.fix_rule()          # the mixed rule is not tuned inside the loop
+        return prep
 
     def issue(self, prep, lo: int, hi: int):
         """Enqueue rows [lo, hi); returns an object whose query() is True once done."""
