@@ -51,18 +51,22 @@ enum { MHS_FOLD = 1, MHS_SUBTRACT = 2, MHS_PARTS = 4, MHS_MIXED = 8 };
  * fold passes add to it.  A partial mask is for timing a pass alone. */
 enum { MHS_STAGE_PAIR = 1, MHS_STAGE_Q = 2, MHS_STAGE_FOLD = 4 };
 
-/* The mixed mode's far-field rule for the contracted interaction (one or two
- * output components).  With MHS_MIXED a (receiver, source) pair is near --
+/* The mixed mode's far-field rule (every form, contracted and tensor passes
+ * and the fold's far points).  With MHS_MIXED a (receiver, source) pair is near --
  * closed forms in FP64 -- when sqrt(|x - c|^2 + eps^2) <= near_ratio L (the
  * reference's D* is 10); otherwise each far-field family (direct on the
  * source triangle; image R and Q on the mirrored one, measured from its
  * centroid) takes the Gauss order order[b] + (nodal order) of the first
  * band b with that distance <= band_hi[b] L (the last band takes the rest),
  * squared for the collapsed triangle rule; image Q pairs that are near keep
- * the budget law.  near_ratio = 0 selects the library's default rule.  The rule
- * spends the mixed mode's error budget (1e-6 of the matrix's largest entry)
- * on fewer points; the all-FP64 mode always uses the reference's rule
- * (fullspace/defaults.py:20-23, kernels/image.py:249-254). */
+ * the budget law.  near_ratio = 0 selects the library's default rule: near
+ * 4 L, orders 5 / 4 / 3 / 2 up to 6 / 16 / 100 L / beyond, whose quadrature
+ * error is below 1e-7 of each pair's own largest entry for every form.  A
+ * caller may pass a cheaper rule where its rows tolerate it (the on-fault
+ * interaction, whose rows each hold their own element's near field); the
+ * mixed mode's bound is 1e-6 of each row's largest entry.  The all-FP64
+ * mode always uses the reference's rule (fullspace/defaults.py:20-23,
+ * kernels/image.py:249-254). */
 #define MHS_RULE_BANDS 8
 typedef struct mhs_rule {
   double near_ratio;
@@ -107,7 +111,7 @@ typedef struct mhs_launch {
   void* fold_work;          /* scratch: fold_work_items x mhs_fold_item_bytes(order) bytes, the
                                folded pairs' polar grids and centre loads (prepared in parallel) */
   int64_t fold_work_items;  /* capacity in folded pairs; more pairs than this set fold_err bit 2 */
-  mhs_rule rule;            /* MHS_MIXED contracted interaction: the far-field rule */
+  mhs_rule rule;            /* MHS_MIXED: the far-field rule (near_ratio 0: the default) */
   const double* near_tiles; /* optional (NULL): per 256 element columns, 8 doubles: the bounding
                                box (lo xyz, hi xyz) of the sources' centroids and mirrored
                                centroids, their largest L, and 1 if any straddles (fold) else

@@ -75,9 +75,12 @@ def _coerce_normals(normals, n_obs):
 
 
 def _prepare_out(out, shape, n_obs, n_src, kind, parts):
-    """matrices.py:169-189: validate out=, or budget-check the allocation."""
+    """matrices.py:169-189: validate out=, or budget-check the allocation
+    (the host ceiling; a device-resident plan, :func:`plan_device`, is
+    admitted against the device's memory by its caller instead)."""
     if out is None:
-        chunking.refuse_if_too_large(n_obs, n_src, kind, np.float64, parts)
+        if not getattr(_tls, "device_only", False):
+            chunking.refuse_if_too_large(n_obs, n_src, kind, np.float64, parts)
         return None
     if not isinstance(out, np.ndarray):
         raise TypeError(f"out must be a numpy array, got {type(out).__name__}")
@@ -237,6 +240,7 @@ def interaction_matrix(tris, material, eps, *, receiver=("strike", "dip", "norma
     buf = _prepare_out(out, shape, hi - lo, n_src_dof, (len(rec), len(src)), False)
     job = _job("interaction", obs, tri, mu, lam, eps_arr, p, ids, sn, fold,
                subtract=bool(subtract_eigenstress),
+               self_rows=sn is not None or (obs_tris is None and obs_pts is None),
                nrm=np.ascontiguousarray(rec_frame[lo:hi, 2, :]),
                rb=np.ascontiguousarray(rec_frame[lo:hi][:, rec, :]),
                sb=np.ascontiguousarray(src_frame[:, src, :]))
@@ -288,4 +292,16 @@ def plan(fn, *args, **kwargs) -> runtime.Job:
     if len(jobs) != 1:
         raise ValueError(f"{getattr(fn, '__name__', fn)!r} is not a single-job matrix function")
     return jobs[0]
+
+
+def plan_device(fn, *args, **kwargs) -> runtime.Job:
+    """:func:`plan` for a result that stays in device memory (the serving
+    loop): the same coercion and checks, but not the host's byte ceiling
+    (chunking.MATRIX_MAX_BYTES bounds host RAM; the caller admits the job
+    against the device's memory, runtime.job_bytes)."""
+    _tls.device_only = True
+    try:
+        return plan(fn, *args, **kwargs)
+    finally:
+        _tls.device_only = False
 

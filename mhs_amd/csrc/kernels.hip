@@ -1408,20 +1408,21 @@ __global__ void k_strain(double* __restrict__ buf, int64_t n_obs, int64_t n_cols
   }
 }
 
-// The mixed mode's default far-field rule: closed forms up to 4 L, then 3 x 3
-// points per family up to 8 L, 2 x 2 up to 60 L and one point beyond (plus
-// the nodal order).  Per family and pair its error is at most ~3e-9 of the
-// largest entry of the strike-slip interaction matrix of a 10,000-triangle
-// fault (measured against the FP64 rule per distance band), within the
-// mixed mode's bound of 1e-6.
+// The mixed mode's default far-field rule: closed forms up to 4 L, then 5 x 5
+// Gauss points per family up to 6 L, 4 x 4 up to 16 L, 3 x 3 up to 100 L and
+// 2 x 2 beyond (plus the nodal order per direction).  Every pair's
+// quadrature error stays below 1e-7 of the pair's own largest entry, for
+// every form and order (worst measured: the traction of a constant-slip
+// source against 24 x 24 Gauss, 6e-8 at 100 L), so a row's error is bounded
+// relative to the row wherever its observer lies.
 #ifndef MHS_FARM_CT
 #define MHS_FARM_CT 2   // default column tiles (of 256 sources) per block of the mixed far pass
 #endif
 static Rule default_rule() {
   Rule r;
   r.near = 4.0;
-  const double hi[MHS_RULE_BANDS] = {8.0, 60.0, 1e300, 1e300, 1e300, 1e300, 1e300, 1e300};
-  const int n[MHS_RULE_BANDS] = {3, 2, 1, 1, 1, 1, 1, 1};
+  const double hi[MHS_RULE_BANDS] = {6.0, 16.0, 100.0, 1e300, 1e300, 1e300, 1e300, 1e300};
+  const int n[MHS_RULE_BANDS] = {5, 4, 3, 2, 2, 2, 2, 2};
   for (int b = 0; b < MHS_RULE_BANDS; ++b) { r.hi[b] = hi[b]; r.n[b] = n[b]; }
   return r;
 }

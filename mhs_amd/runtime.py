@@ -28,24 +28,38 @@ def tile_candidates(mixed: bool):
     return FAR_TILE_CANDIDATES_MIXED if mixed else FAR_TILE_CANDIDATES
 
 
-#: The mixed mode's far-field rule for the contracted interaction, as
-#: (near_ratio, ((band_hi, order), ...)) in units of the source's size L:
-#: None lets the autotuner choose among RULE_CANDIDATES (the library's
-#: default when autotuning is off).  See include/mhs_amd.h (mhs_rule).
+#: The mixed mode's far-field rule, as (near_ratio, ((band_hi, order), ...))
+#: in units of the source's size L (Gauss points per direction: order + the
+#: shape order): None lets the autotuner choose among RULE_CANDIDATES for an
+#: on-fault interaction and takes the library's default, RULE_SAFE, for
+#: everything else.  See include/mhs_amd.h (mhs_rule).
 mixed_rule = None
 
-#: Candidate rules of the tuner, from the fewest points to the most.  Each
-#: family's error per distance band was measured against the FP64 rule on the
-#: strike-slip interaction of a 10,000-triangle fault: 3e-9 of the largest
-#: entry or less per family for the first, less for the others.
+#: The library's default mixed rule: every pair's quadrature error below
+#: 1e-7 of the pair's own largest entry, for every form and shape order
+#: (worst case measured: the traction of a constant-slip source, 2 x 1
+#: triangles at 20-90 deg dip, against 24 x 24 Gauss: at most 5e-10 at 4-6 L
+#: with order 5, 7e-9 at 6-16 L with 4, 8e-9 at 16-100 L with 3 and 6e-8
+#: beyond with 2).
+#: So each row's error is bounded relative to the row, wherever its observer.
+RULE_SAFE = (4.0, ((6.0, 5), (16.0, 4), (100.0, 3), (1e300, 2)))
+
+#: Candidate rules of the tuner for the on-fault interaction, from the fewest
+#: points to the most.  Their low orders far away (one point per direction
+#: beyond 60 L in the first) are far less accurate per pair than RULE_SAFE,
+#: but the receiver rows of an on-fault interaction each hold their own
+#: element's near field, which the far pairs' errors (decaying with the
+#: stress kernel's 1 / r^3) stay far below: the tuner measures each row's
+#: error relative to the row and keeps only candidates within the bound.
 RULE_CANDIDATES = (
     (4.0, ((8.0, 3), (60.0, 2), (1e300, 1))),
     (4.0, ((10.0, 3), (100.0, 2), (1e300, 1))),
     (5.0, ((12.0, 4), (30.0, 3), (150.0, 2), (1e300, 1))),
-    (6.0, ((16.0, 5), (40.0, 4), (100.0, 3), (1e300, 2))),
+    RULE_SAFE,
 )
-#: The tuner's bound on a candidate's error over the sample rows, relative to
-#: the sample's largest FP64 entry (the mixed mode's stated bound is 1e-6).
+#: The tuner's bound on a candidate's error: the largest over the sample rows
+#: of max |mixed - FP64| / max |FP64| in the row (the mixed mode's stated
+#: bound is 1e-6 per row).
 RULE_ERROR_BOUND = 1e-7
 
 
@@ -129,6 +143,9 @@ class Job:
     sb: np.ndarray | None = None       # (n_src | n_nodes, n_b, 3)
     ids: np.ndarray | None = None      # (n_src, K) continuous columns
     n_nodes: int = 0
+    # receivers are the source mesh's own collocation points (or shared
+    # nodes): every row holds its own element's near field
+    self_rows: bool = False
     extra: dict = field(default_factory=dict)
 
     @property
@@ -345,6 +362,7 @@ class Prepared:
             with torch.cuda.stream(self.stream):
                 self.near_tiles_t = dev.put(near_tile_bounds(recs, K))
         self.far_tiles = 0
+        self.tiles_fixed = False
         self.rule = rule_struct(mixed_rule)
         self.rule_tuned = False
 
@@ -389,28 +407,41 @@ class Prepared:
         return sorted({(max(0, min(n - per, int(f * n) - per // 2)),
                         max(0, min(n - per, int(f * n) - per // 2)) + per) for f in (1 / 6, 1 / 2, 5 / 6)})
 
+    def rule_key(self):
+        """The class a tuned mixed rule is cached under: form, whether the
+        rows are the mesh's own collocation points, order, outputs, size."""
+        order, outputs, _, cols = self.tune_key()
+        return (self.job.kind, bool(self.job.self_rows), order, outputs, cols)
+
     def fix_rule(self):
         """Use the rule fixed in ``mixed_rule``, else the device's tuned rule
         for this job's class if there is one, else the library default --
         without tuning (for callers that must not synchronise)."""
-        key = (self.job.kind,) + self.tune_key()
-        self.rule = rule_struct(mixed_rule if mixed_rule is not None else self.dev.rules.get(key))
+        self.rule = rule_struct(mixed_rule if mixed_rule is not None else self.dev.rules.get(self.rule_key()))
         self.rule_tuned = True
+
+    def fix_tiles(self):
+        """Use the device's tuned far-pass tiles for this job's class if there
+        are any, else the library default (``far_tiles`` 0), without tuning."""
+        self.far_tiles = self.dev.far_tiles.get(self.tune_key(), 0) if self.contracted else 0
+        self.tiles_fixed = True
 
     def tune_rule(self):
         """Choose the mixed mode's far-field rule (near/far switch and Gauss
         orders by distance band) under a measured error bound: on the sample
         rows the FP64 rule's passes (far, near, fold) are the reference; every
         candidate of RULE_CANDIDATES is run in the mixed mode on the same
-        rows, its error max|mixed - FP64| / max|FP64| measured and its passes
-        timed (HIP events, after one warm launch); the fastest candidate
-        within RULE_ERROR_BOUND wins (none within: the most conservative).
-        Cached per (form, order, outputs, size class) on the device with
-        every candidate's error and time.  A rule fixed in ``mixed_rule`` is
-        used untuned; so is the library's default for the continuous layout
-        and for jobs smaller than the sample (AUTOTUNE_PAIRS pairs), which a
-        tuned rule of their class replaces once one exists.  Synchronises
-        the stream."""
+        rows, its error (the largest over the rows of max|mixed - FP64| /
+        max|FP64| in the row) measured and its passes timed (HIP events,
+        after one warm launch); the fastest candidate within
+        RULE_ERROR_BOUND wins (none within: the last, RULE_SAFE).  Cached per
+        rule_key() on the device with every candidate's error and time.
+        Only the on-fault interaction (``job.self_rows``) is tuned: other
+        forms take the library's default, RULE_SAFE, whose error is bounded
+        per pair.  A rule fixed in ``mixed_rule`` is used untuned; so is the
+        library's default for the continuous layout and for jobs smaller
+        than the sample (AUTOTUNE_PAIRS pairs), which a tuned rule of their
+        class replaces once one exists.  Synchronises the stream."""
         if not self.job.mixed or self.rule_tuned:
             return
         self.rule_tuned = True
@@ -418,11 +449,11 @@ class Prepared:
         if mixed_rule is not None or not autotune:
             self.rule = rule_struct(mixed_rule)
             return
-        key = (self.job.kind,) + self.tune_key()
+        key = self.rule_key()
         if key in dev.rules:
             self.rule = rule_struct(dev.rules[key])
             return
-        if self.cont or self.job.n_obs * self.n_cols < AUTOTUNE_PAIRS:
+        if not self.job.self_rows or self.cont or self.job.n_obs * self.n_cols < AUTOTUNE_PAIRS:
             self.rule = rule_struct(None)
             return
         blocks = self._sample_blocks()
@@ -443,7 +474,8 @@ class Prepared:
         with torch.cuda.stream(self.stream):
             run(self.flags & ~_lib.F_MIXED, _lib.Rule(), 0)
             ref = [self.final[lo:hi].clone() for lo, hi in blocks]
-            scale = max(float(r.abs().max()) for r in ref) or 1.0
+            red = tuple(range(1, self.final.dim()))
+            scale = [r.abs().amax(dim=red).clamp_min(1e-300) for r in ref]
             for cand in RULE_CANDIDATES:
                 rs = rule_struct(cand)
                 run(self.flags, rs, 0)
@@ -452,7 +484,8 @@ class Prepared:
                 run(self.flags, rs, 0)
                 e1.record(self.stream)
                 e1.synchronize()
-                err = max(float((self.final[lo:hi] - r).abs().max()) for (lo, hi), r in zip(blocks, ref)) / scale
+                err = max(float(((self.final[lo:hi] - r).abs().amax(dim=red) / sc).max())
+                          for (lo, hi), r, sc in zip(blocks, ref, scale))
                 table[cand] = (err, e0.elapsed_time(e1))
         ok = [c for c, (err, _) in table.items() if err <= RULE_ERROR_BOUND]
         best = min(ok, key=lambda c: table[c][1]) if ok else RULE_CANDIDATES[-1]
@@ -512,7 +545,7 @@ class Prepared:
         of them in the continuous layout); no host synchronisation (except
         the autotuner's first visit of a new launch shape)."""
         job, lib = self.job, self.dev.lib
-        if self.contracted and not self.far_tiles:
+        if self.contracted and not self.far_tiles and not self.tiles_fixed:
             self.tune()
         if job.mixed and not self.rule_tuned:
             self.tune_rule()

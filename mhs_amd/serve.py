@@ -80,27 +80,40 @@ SCRATCH_HEADROOM_BYTES = 16 << 30
 
 class GpuBackend:
     """Device side of the loop: one stream (default priority) per slot, the
-    HIP kernels through :class:`runtime.Prepared`, completion by events."""
+    HIP kernels through :class:`runtime.Prepared`, completion by events.
 
-    def __init__(self, slots: int, dev: runtime.Device | None = None):
+    With ``tune`` False (the default) a request uses the device's tuned
+    mixed rule and far-pass tiles for its class if an earlier run tuned
+    them, else the library defaults, so the loop never waits on a tuner;
+    with ``tune`` True the first request of each class tunes them (and
+    synchronises its stream), as a warm-up does."""
+
+    def __init__(self, slots: int, dev: runtime.Device | None = None, headroom: int = SCRATCH_HEADROOM_BYTES,
+                 tune: bool = False):
         self.dev = dev or runtime.device()
         self.torch = self.dev.torch
         self.streams = [self.torch.cuda.Stream(device=self.dev.dev) for _ in range(slots)]
         self.total = int(self.torch.cuda.get_device_properties(self.dev.dev).total_memory)
+        self.headroom = int(headroom)
+        self.tune = bool(tune)
+        self.trims = 0          # times the cache was returned to the device
 
     def default_budget(self) -> int:
         return int(0.8 * self.total)
 
     def trim(self) -> None:
         """Return the tensor cache's unused blocks to the device when it has
-        reserved more than all but SCRATCH_HEADROOM_BYTES of it (requests of
+        reserved more than all but ``headroom`` bytes of it (requests of
         many sizes fragment the cache until it holds nearly all of HBM)."""
-        if self.torch.cuda.memory_reserved(self.dev.dev) > self.total - SCRATCH_HEADROOM_BYTES:
+        if self.torch.cuda.memory_reserved(self.dev.dev) > self.total - self.headroom:
             self.torch.cuda.empty_cache()
+            self.trims += 1
 
     def prepare(self, job: runtime.Job, slot: int):
         prep = runtime.Prepared(job, stream=self.streams[slot], dev=self.dev)
-        prep.fix_rule()          # the mixed rule is not tuned inside the loop
+        if not self.tune:
+            prep.fix_rule()
+            prep.fix_tiles()
         return prep
 
     def issue(self, prep, lo: int, hi: int):
@@ -133,12 +146,19 @@ class Server:
         self.depth = int(depth)
 
     def _plan(self, req: Request) -> runtime.Job:
+        # results stay on the device: admitted against the serving budget
+        # below, not the host's byte ceiling
         with runtime.precision(req.precision):
-            return api.plan(req.fn, *req.args, **req.kwargs)
+            return api.plan_device(req.fn, *req.args, **req.kwargs)
 
-    def run(self, requests: list[Request], keep: bool = True, realtime: bool = True) -> list[Served]:
+    def run(self, requests: list[Request], keep: bool = True, realtime: bool = True,
+            consume: Callable | None = None) -> list[Served]:
         """Serve ``requests`` (arrival times relative to the call); returns one
-        :class:`Served` per request, in input order."""
+        :class:`Served` per request, in input order.  ``consume(index, out,
+        out2)``, if given, is called as each request completes (after its
+        ``done`` time is taken) with its device results, and what it returns
+        is kept in ``Served.out`` in place of the results, which are then
+        released: a consumer that copies or reduces each matrix as it lands."""
         be = self.backend
         order = sorted(range(len(requests)), key=lambda i: requests[i].arrival)
         served = [Served(i, r.tag, float(r.arrival)) for i, r in enumerate(requests)]
@@ -198,7 +218,9 @@ class Server:
                 s.done = time.perf_counter() - t0
                 try:
                     out, out2 = be.finish(a.prep)
-                    if keep:
+                    if consume is not None:
+                        s.out = consume(a.index, out, out2)
+                    elif keep:
                         s.out, s.out2 = out, out2
                 except RuntimeError as e:
                     s.error = str(e)
@@ -232,9 +254,6 @@ def summary(served: list[Served]) -> dict:
 
 
 # ------------------------------------------------------------ request mix --
-INTERACTION_MAX_DOFS = 12_000
-
-
 def c5_requests(n_req: int = 512, rate: float = 1.0, precision: str = "fp64",
                 seed: int | None = None) -> list[Request]:
     """The served stream: ``n_req`` requests with Poisson arrivals at ``rate``
@@ -243,9 +262,11 @@ def c5_requests(n_req: int = 512, rate: float = 1.0, precision: str = "fp64",
     U[0, 5] km, order U{0, 1, 2}); 40 % ``disp_matrix`` on a 32 x 32 surface
     grid over [-30, 30]^2, 30 % ``traction_matrix`` at 2000 observers uniform
     in the fault's box widened by 10 km (z <= 0) with random unit normals,
-    30 % ``interaction_matrix`` strike on strike.  An interaction mesh is cut
-    to its first INTERACTION_MAX_DOFS / K triangles so that no single matrix
-    passes 1.2 GB (the uncut order-2 tail reaches 48,000^2 entries)."""
+    30 % ``interaction_matrix`` with the reference's default components
+    (receiver strike / dip / normal, source strike / dip / tensile).  Meshes
+    are not cut: the order-2 tail's interaction matrices reach 48,000^2 x 9
+    entries (166 GB), admitted against the device's memory, not the host's
+    byte ceiling."""
     from . import api, workloads
     from .materials import Material
     rng = np.random.default_rng(workloads.SEED + 5 if seed is None else seed)
@@ -271,9 +292,7 @@ def c5_requests(n_req: int = 512, rate: float = 1.0, precision: str = "fp64",
             nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
             req = Request(api.traction_matrix, (obs, nrm, tris, mat, eps), {"order": p})
         else:
-            tris = tris[: max(1, INTERACTION_MAX_DOFS // K)]
-            req = Request(api.interaction_matrix, (tris, mat, eps),
-                          {"order": p, "receiver": "strike", "source": "strike"})
+            req = Request(api.interaction_matrix, (tris, mat, eps), {"order": p})
         req.arrival, req.precision = t, precision
         req.tag = f"{r['kind']} n={tris.shape[0]} p={p}"
         out.append(req)

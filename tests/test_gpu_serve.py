@@ -58,3 +58,192 @@ def test_served_equals_alone():
         assert torch.equal(s.out, a), (r.fn.__name__, r.precision, float((s.out - a).abs().max()))
     summ = serve.summary(served)
     assert summ["completed"] == len(reqs)
+
+
+# --------------------------------------------- a slice of the served stream --
+def _stream_slice():
+    """Requests of the seeded served stream (uncut meshes, default 3 x 3
+    interaction components): the smallest mesh, the largest disp/traction
+    mesh, the largest interaction that fits 16 GB, and the first request of
+    every (form, order) not yet covered among those under 4 GB."""
+    from mhs_amd import api, runtime, serve
+    reqs = serve.c5_requests(n_req=64, rate=1.0, seed=11)
+    info = []
+    for i, r in enumerate(reqs):
+        job = api.plan_device(r.fn, *r.args, **r.kwargs)
+        info.append((i, job.kind, job.n_src, job.order, runtime.job_bytes(job)))
+    pick = {min(info, key=lambda t: t[2])[0],
+            max((t for t in info if t[1] != "interaction"), key=lambda t: t[2])[0],
+            max((t for t in info if t[1] == "interaction" and t[4] <= 16e9), key=lambda t: t[2])[0]}
+    covered = {(info[i][1], info[i][3]) for i in pick}
+    for t in info:
+        if (t[1], t[3]) not in covered and t[4] <= 4e9:
+            pick.add(t[0])
+            covered.add((t[1], t[3]))
+    return [reqs[i] for i in sorted(pick)]
+
+
+def _points(req, K):
+    if req.fn.__name__ != "interaction_matrix":
+        return req.args[0]
+    from mhs_amd.api import collocation_points
+    return collocation_points(req.args[0], {1: 0, 3: 1, 6: 2}[K])
+
+
+def _samples(req, K, n_rows):
+    """Rows 0 and n_rows // 2, and the source triangles nearest to them plus
+    a seeded random set (near, far and fold pairs)."""
+    tris = req.args[-3]
+    rows = [0, n_rows // 2]
+    rng = np.random.default_rng(n_rows)
+    cen = tris.mean(axis=1)
+    pts = _points(req, K)
+    pick = set(rng.choice(tris.shape[0], size=min(16, tris.shape[0]), replace=False).tolist())
+    for r in rows:
+        pick.update(np.argsort(np.linalg.norm(cen - pts[r], axis=1))[:12].tolist())
+    src = np.array(sorted(pick))
+    dof = (src[:, None] * K + np.arange(K)).ravel()
+    return rows, src, dof
+
+
+def _oracle_block(req, rows, src):
+    from oracle import api as O
+    name, a, p = req.fn.__name__, req.args, req.kwargs["order"]
+    if name == "interaction_matrix":
+        tris, mat, eps = a
+        return np.concatenate([O.interaction_matrix(tris[src], mat.mu, mat.lam, eps, obs_tris=tris, order=p,
+                                                    rows=(r, r + 1)) for r in rows])
+    if name == "disp_matrix":
+        obs, tris, mat, eps = a
+        return O.disp_matrix(obs[rows], tris[src], mat.mu, mat.lam, eps, order=p)
+    obs, nrm, tris, mat, eps = a
+    return O.traction_matrix(obs[rows], nrm[rows], tris[src], mat.mu, mat.lam, eps, order=p)
+
+
+#: Distance ratios D / L (D = sqrt(|x - centroid|^2 + eps^2), L the longest
+#: edge, for the triangle and for its mirror image) over which the closed-form
+#: moments run (D <= 10 L) but lose digits to cancellation as D / L grows: at
+#: order 1 and 2 the oracle's own rounding error there reaches 3e-8 of a
+#: source column's scale, so GPU and oracle can only agree to their rounding.
+ILL_CONDITIONED = (2.5, 10.0)
+
+
+def _ill_mask(req, rows, src, K, shape):
+    """True for the entries of a sampled block whose pair lies in the
+    ill-conditioned band of the closed-form near field."""
+    from oracle.geometry import frame_of
+    tris, eps = req.args[-3], float(req.args[-1])
+    pts = _points(req, K)[rows]
+    m = np.zeros((len(rows), len(src)), bool)
+    for j, s in enumerate(src):
+        for tri in (tris[s], tris[s] * np.array([1.0, 1.0, -1.0])):
+            fr = frame_of(tri)
+            dl = np.sqrt(np.sum((pts - fr.centroid) ** 2, axis=1) + eps * eps) / fr.L
+            m[:, j] |= (dl >= ILL_CONDITIONED[0]) & (dl <= ILL_CONDITIONED[1])
+    m = np.repeat(m, K, axis=1)
+    return m[:, :, None, None] if req.fn.__name__ == "interaction_matrix" else m[:, None, :, None]
+
+
+def _truth_block(req, rows, src):
+    """The sampled block with the moments of every pair beyond 2.5 L from
+    24 x 24 Gauss quadrature (converged to ~1e-14 of the column's scale
+    there, where the integrand is smooth) and the closed forms inside it:
+    the reference against which the two FP64 closed forms' rounding is
+    compared in the ill-conditioned band."""
+    import oracle.moments as OM
+    saved = OM.D_STAR, OM.FAR_GAUSS_N, OM.FAR_GAUSS_N_DISTANT
+    OM.D_STAR, OM.FAR_GAUSS_N, OM.FAR_GAUSS_N_DISTANT = ILL_CONDITIONED[0], 24, 24
+    try:
+        return _oracle_block(req, rows, src)
+    finally:
+        OM.D_STAR, OM.FAR_GAUSS_N, OM.FAR_GAUSS_N_DISTANT = saved
+
+
+@pytest.fixture(scope="module")
+def stream_slice():
+    from mhs_amd import api
+    cases = []
+    for r in _stream_slice():
+        job = api.plan_device(r.fn, *r.args, **r.kwargs)
+        rows, src, dof = _samples(r, job.K, job.n_obs)
+        ref = _oracle_block(r, rows, src)
+        ill = np.broadcast_to(_ill_mask(r, rows, src, job.K, ref.shape), ref.shape)
+        truth = _truth_block(r, rows, src) if job.order > 0 else None
+        cases.append((r, rows, dof, ref, ill, truth))
+    return cases
+
+
+def _consumer(cases, torch):
+    """Per request: an exact digest of every bit of the result (an int64 sum
+    of the words) and the sampled block, copied to the host."""
+    def consume(i, out, out2):
+        rows, dof = cases[i][1], cases[i][2]
+        dig = int(out.view(torch.int64).sum())
+        if out2 is not None:
+            dig ^= int(out2.view(torch.int64).sum())
+        blk = out[rows]
+        blk = blk[:, dof] if cases[i][0].fn.__name__ == "interaction_matrix" else blk[:, :, dof]
+        return dig, blk.cpu().numpy()
+    return consume
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_stream_slice_alone_mix_and_oracle(stream_slice, prec):
+    """Uncut meshes of the served stream (about 100 to 7,000 triangles, every
+    form and order): each result is bitwise the same served alone (one slot,
+    one chunk) and in the mix (four slots, small row chunks, results
+    consumed as they land, the cache trimmed whenever it holds more than
+    1 GiB), and its sampled rows match the FP64 oracle: FP64 to 1e-12 of
+    the block's largest entry, except at order 1 and 2 in the closed forms'
+    ill-conditioned band, where both are rounding of the same closed forms
+    and its error against converged quadrature is of the oracle's own size
+    (at most twice it, plus 1e-12); mixed to 1e-6 of each row's largest
+    entry (against the quadrature in the band, where it is the better
+    reference)."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from dataclasses import replace
+    from mhs_amd import serve
+    cases = [(replace(c[0], precision=prec, arrival=0.0),) + c[1:] for c in stream_slice]
+    reqs = [c[0] for c in cases]
+    consume = _consumer(cases, torch)
+    # a warm-up pass tunes each class's far tiles (and the on-fault
+    # interaction's mixed rule); the runs below use the tuned entries
+    warm = serve.Server(slots=1, backend=serve.GpuBackend(1, tune=True)).run(reqs, keep=False, realtime=False)
+    assert all(s.error is None for s in warm)
+    alone = []
+    for i, r in enumerate(reqs):
+        print(f"case alone {prec} {r.tag}", flush=True)
+        solo = serve.Server(slots=1, chunk_pairs=1 << 40, backend=serve.GpuBackend(1))
+        res = solo.run([r], realtime=False, consume=lambda _, o, o2: consume(i, o, o2))[0]
+        assert res.error is None, (r.tag, res.error)
+        alone.append(res.out)
+        torch.cuda.empty_cache()
+    print(f"case mix {prec} of {len(reqs)}", flush=True)
+    be = serve.GpuBackend(4)
+    be.headroom = be.total - (1 << 30)
+    served = serve.Server(slots=4, chunk_pairs=1 << 22, backend=be).run(reqs, realtime=False, consume=consume)
+    assert serve.summary(served)["completed"] == len(reqs)
+    assert be.trims > 0
+    for (r, _, _, ref, ill, truth), a, s in zip(cases, alone, served):
+        assert s.out[0] == a[0], (r.tag, "differs alone and in the mix")
+        assert np.array_equal(s.out[1], a[1]), r.tag
+        got, scale = a[1], np.abs(ref).max()
+        if prec == "mixed":
+            # per row, against the oracle (converged quadrature in the band)
+            best = np.where(ill, truth, ref) if truth is not None else ref
+            err = max(float(np.abs(got[j] - best[j]).max() / np.abs(best[j]).max()) for j in range(len(got)))
+            print(f"mixed {r.tag}: row relmax vs oracle {err:.3e}", flush=True)
+            assert err < 1e-6, (r.tag, err)
+            continue
+        well = ~ill if truth is not None else np.ones(ref.shape, bool)
+        err = float(np.abs(got - ref)[well].max() / scale) if well.any() else 0.0
+        msg = f"fp64 {r.tag}: relmax vs oracle {err:.3e}"
+        if truth is not None and ill.any():
+            e_gpu = float(np.abs(got - truth)[ill].max() / scale)
+            e_ora = float(np.abs(ref - truth)[ill].max() / scale)
+            msg += f"; band {ILL_CONDITIONED[0]}-{ILL_CONDITIONED[1]} L: gpu {e_gpu:.2e}, oracle {e_ora:.2e} vs quadrature"
+            assert e_gpu <= 2.0 * e_ora + 1e-12, (r.tag, e_gpu, e_ora)
+        print(msg, flush=True)
+        assert err < 1e-12, (r.tag, err)

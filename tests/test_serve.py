@@ -120,10 +120,28 @@ def test_c5_stream_shapes():
     arr = np.array([r.arrival for r in reqs])
     assert np.all(np.diff(arr) > 0)
     for r in reqs:
-        job = M.api.plan(r.fn, *r.args, **r.kwargs)
+        job = M.api.plan_device(r.fn, *r.args, **r.kwargs)
         if job.kind == "interaction":
-            assert job.n_obs <= serve.INTERACTION_MAX_DOFS
+            # uncut meshes, the reference's default 3 x 3 components
+            assert job.n_obs == job.n_src * job.K and (job.n_a, job.n_b) == (3, 3)
         assert runtime.job_bytes(job) > 0
+
+
+def test_device_plans_skip_the_host_ceiling_only():
+    """A served (device-resident) request is admitted against the device
+    budget: plan_device skips the host byte ceiling that plan (and the
+    public call) keeps, with the reference's MemoryError."""
+    from mhs_amd import defaults
+    P = workloads.plane(40.0, 20.0, 80, 40, 60.0, -1.0)
+    tris = workloads.orient_up(workloads.grid_mesh(P))        # 6,400 triangles
+    mat = M.Material(30.0, 30.0)
+    kw = {"order": 2}                                       # (38400, 38400, 3, 3): 106 GB
+    with pytest.raises(MemoryError):
+        M.api.plan(M.interaction_matrix, tris, mat, 0.1, **kw)
+    job = M.api.plan_device(M.interaction_matrix, tris, mat, 0.1, **kw)
+    assert runtime.job_bytes(job) > defaults.MATRIX_MAX_BYTES
+    with pytest.raises(MemoryError):                        # the flag does not leak into later calls
+        M.api.plan(M.interaction_matrix, tris, mat, 0.1, **kw)
 
 
 def test_cache_is_trimmed_as_requests_finish():
