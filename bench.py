@@ -34,54 +34,112 @@ PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}     # MI355X vector FP64, FP32 (publis
 CPU_SEED = 20261203
 
 
-def _cpu_block(cols):
+#: The reference's published single-core figure (README "Across cores"): the
+#: strike/strike interaction of an 800-element fault, 640k pairs in 30.1 s.
+PUBLISHED_US_PER_PAIR = 47.1
+#: Worker processes for the CPU baseline: the host's CPU share for one GPU.
+CPU_SHARE = 16
+
+
+def _cpu_block(task):
+    """One task of the CPU baseline: ``(workload, columns)``; returns
+    (pairs, CPU seconds of this process for the task)."""
     from mhs_amd import workloads
     from oracle import api as oapi
-    tris, eps = workloads.c3_interaction()
+    what, cols = task
+    t0 = time.process_time()
+    if what == "c3":
+        tris, eps = workloads.c3_interaction()
+    else:           # the published benchmark's size: a buried 800-element plane
+        tris, eps = workloads.orient_up(workloads.grid_mesh(workloads.plane(40.0, 20.0, 20, 20, 60.0, -2.0))), 0.1
     K = oapi.interaction_matrix(tris[cols], 30.0, 30.0, eps, receiver="strike", source="strike",
                                 obs_tris=tris)
-    return int(K.shape[0] * K.shape[1])
+    return int(K.shape[0] * K.shape[1]), time.process_time() - t0
 
 
-def cpu_baseline(n_cols: int, cores: int) -> dict:
-    """The FP64 oracle on all 10^4 receivers x ``n_cols`` seeded source
-    columns (the reference's own loop runs per source, vectorised over the
-    receivers), one column per task on a pool of one spawned process per core
-    (BLAS pinned to one thread), the surface-reaching (folded, costliest)
-    columns first; pair-evals/s over the wall time.  The sample is
-    stratified: surface-reaching sources (shallowest vertex above 3 eps
-    depth) and the rest are drawn in proportion to their counts."""
+def _pool_run(tasks, procs):
+    """Run tasks on a pool of ``procs`` spawned processes with BLAS pinned to
+    one thread; returns ([(pairs, cpu_s)] in task order, wall seconds)."""
     import multiprocessing as mp
-    from mhs_amd import workloads
-    rng = np.random.default_rng(CPU_SEED)
-    tris, eps = workloads.c3_interaction()
-    shallow = tris[:, :, 2].max(axis=1) > -3.0 * eps
-    strad, rest = np.flatnonzero(shallow), np.flatnonzero(~shallow)
-    n_sh = int(round(n_cols * len(strad) / len(tris)))
-    cols = np.concatenate([rng.choice(strad, size=n_sh, replace=False),
-                           rng.choice(rest, size=n_cols - n_sh, replace=False)])
-    blocks = [np.array([c]) for c in cols]
     env = {"OMP_NUM_THREADS": "1", "OPENBLAS_NUM_THREADS": "1", "MKL_NUM_THREADS": "1"}
     old = {k: os.environ.get(k) for k in env}
     os.environ.update(env)
     try:
-        ctx = mp.get_context("spawn")
-        with ctx.Pool(min(cores, len(blocks))) as pool:
+        with mp.get_context("spawn").Pool(procs) as pool:
             t0 = time.perf_counter()
-            pairs = sum(pool.imap_unordered(_cpu_block, blocks, chunksize=1))
-            dt = time.perf_counter() - t0
+            res = pool.map(_cpu_block, tasks, chunksize=1)
+            return res, time.perf_counter() - t0
     finally:
         for k, v in old.items():
             if v is None:
                 os.environ.pop(k, None)
             else:
                 os.environ[k] = v
-    used = min(cores, len(blocks))
-    return {"value": pairs / dt, "unit": "pair-evals/s", "cores": used, "kind": "port",
+
+
+def _physical_cores():
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True, timeout=10).stdout
+        return len({ln for ln in out.splitlines() if ln and not ln.startswith("#")}) or None
+    except (OSError, subprocess.SubprocessError):
+        return None
+
+
+def cpu_baseline(n_cols: int) -> dict:
+    """The FP64 oracle (a numpy restatement of the reference, which runs one
+    source column at a time vectorised over the receivers) on all 10^4
+    receivers x ``n_cols`` seeded source columns of the workload, one column
+    per task on CPU_SHARE spawned processes (BLAS pinned to one thread),
+    surface-reaching (folded, costliest) columns first; ``value`` is the
+    sample's pairs over its wall time.  The sample is stratified (surface-
+    reaching sources, shallowest vertex above 3 eps depth, and the rest, in
+    proportion to their counts), each task's CPU seconds are kept per
+    stratum, and the whole 10^4 x 10^4 matrix is extrapolated from them.
+    Also: a one-process run (one column per stratum) and the restatement's
+    serial us/pair on the published benchmark's size beside the published
+    figure."""
+    from mhs_amd import workloads
+    rng = np.random.default_rng(CPU_SEED)
+    tris, eps = workloads.c3_interaction()
+    n = tris.shape[0]
+    shallow = tris[:, :, 2].max(axis=1) > -3.0 * eps
+    strad, rest = np.flatnonzero(shallow), np.flatnonzero(~shallow)
+    n_sh = int(round(n_cols * len(strad) / n))
+    cols = np.concatenate([rng.choice(strad, size=n_sh, replace=False),
+                           rng.choice(rest, size=n_cols - n_sh, replace=False)])
+    procs = min(CPU_SHARE, len(cols))
+    res, wall = _pool_run([("c3", np.array([c])) for c in cols], procs)
+    pairs = sum(p for p, _ in res)
+    per = {}
+    for name, sel, count in (("surface_reaching", slice(0, n_sh), len(strad)),
+                             ("buried", slice(n_sh, None), len(rest))):
+        p = sum(x for x, _ in res[sel])
+        c = sum(t for _, t in res[sel])
+        per[name] = {"columns_sampled": len(res[sel]), "columns_in_matrix": int(count),
+                     "cpu_us_per_pair": 1e6 * c / max(p, 1)}
+    total_cpu = sum(v["columns_in_matrix"] * n * v["cpu_us_per_pair"] * 1e-6 for v in per.values())
+    one, one_wall = _pool_run([("c3", np.array([cols[0]])), ("c3", np.array([cols[-1]]))], 1)
+    pub, _ = _pool_run([("plane800", np.arange(20))], 1)
+    phys = _physical_cores()
+    return {"value": pairs / wall, "unit": "pair-evals/s", "cores": procs, "kind": "port",
             "sample": f"all 10,000 receivers x {n_cols} seeded source columns (seed {CPU_SEED}; {n_sh} of them "
                       f"surface-reaching, as in the whole mesh) of the same workload, FP64 oracle, "
-                      f"{pairs:.3e} pairs (their fold pairs included) in "
-                      f"{dt:.1f} s wall on {used} processes"}
+                      f"{pairs:.3e} pairs (their fold pairs included) in {wall:.1f} s wall on {procs} "
+                      f"processes (the host's CPU share for one GPU)",
+            "host": {"os_cpu_count": os.cpu_count(), "lscpu_physical_cores": phys},
+            "strata": per,
+            "one_process": {"columns": 2, "pairs": sum(p for p, _ in one), "wall_s": one_wall,
+                            "us_per_pair": 1e6 * one_wall / sum(p for p, _ in one)},
+            "extrapolated": {"label": "extrapolated from the per-stratum CPU seconds, not measured",
+                             "full_matrix_cpu_s": total_cpu,
+                             "full_matrix_wall_s_on_cores": total_cpu / procs,
+                             "pair_evals_per_s_on_cores": n * n / (total_cpu / procs),
+                             "pair_evals_per_s_on_physical_cores": (n * n / (total_cpu / phys)) if phys else None},
+            "vs_published": {"published_us_per_pair": PUBLISHED_US_PER_PAIR,
+                             "port_us_per_pair": 1e6 * pub[0][1] / pub[0][0],
+                             "case": "strike/strike interaction, buried 800-element plane, 20 source columns, "
+                                     "one process (the published case: 800 elements, one core)"}}
 
 
 def _traffic(mode: str):
@@ -258,25 +316,45 @@ def _progress(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def served_stream(n_req: int, precision: str, n_cal: int = 48) -> dict:
-    """The served stream: calibrate E[service] on the first ``n_cal`` requests
-    run one at a time, set the Poisson rate to 0.7 / E[service], then serve
-    ``n_req`` requests through the continuous-batching loop (4 slots, results
-    dropped after completion); p50 / p99 latency and pair-evals/s."""
+#: Poisson arrival rates of the served stream, frozen: 0.7 / E[service] per
+#: mode, E[service] from a per-(form, order) linear cost model (seconds per
+#: request + seconds per pair-eval) fitted to the first 128 requests served
+#: one at a time on an MI355X and applied to all 512 (mixed 0.0807 s, FP64
+#: 0.175 s a request).
+SERVE_RATE = {"mixed": 8.68, "fp64": 4.00}
+#: The stated p99 latency targets.  The stream's size tail sets them: its
+#: largest request (an order-2 3 x 3 interaction of a 5,776-triangle mesh,
+#: 86 GB of output) takes 3.8 s mixed and 9.3 s FP64 alone; a processor-
+#: sharing simulation of the stream on the cost model gives p99 5.0 s and
+#: 13.7 s, and the targets leave 20 % above that.
+SERVE_P99_TARGET_S = {"mixed": 6.0, "fp64": 16.5}
+SERVE_WARM = 48      # requests served one at a time first (tuning the classes they visit)
+
+
+def served_stream(n_req: int, precision: str) -> dict:
+    """The served stream at its frozen rate: the first SERVE_WARM requests
+    served one at a time with tuning on (a warm-up, untimed), then ``n_req``
+    requests with Poisson arrivals at SERVE_RATE through the continuous-
+    batching loop (4 slots, uncut meshes admitted against the device budget,
+    results dropped after completion); p50 / p99 latency, pair-evals/s and
+    the p99 target's pass/fail."""
     from dataclasses import replace
     from mhs_amd import serve
-    _progress(f"served stream: calibrating on {n_cal} requests")
-    cal = serve.c5_requests(n_req=n_cal, rate=1.0, precision=precision)
-    res = serve.Server(slots=1).run([replace(r, arrival=0.0) for r in cal], keep=False, realtime=False)
-    service = float(np.mean([s.done - s.admitted for s in res if s.error is None]))
-    rate = 0.7 / service
-    _progress(f"served stream: {n_req} requests at {rate:.2f}/s")
+    rate = SERVE_RATE[precision]
     reqs = serve.c5_requests(n_req=n_req, rate=rate, precision=precision)
-    out = serve.Server(slots=4).run(reqs, keep=False)
+    _progress(f"served stream {precision}: warm-up on {SERVE_WARM} requests")
+    warm = serve.Server(slots=1, backend=serve.GpuBackend(1, tune=True))
+    warm.run([replace(r, arrival=0.0) for r in reqs[:SERVE_WARM]], keep=False, realtime=False)
+    _progress(f"served stream {precision}: {n_req} requests at {rate:.2f}/s")
+    be = serve.GpuBackend(4)
+    srv = serve.Server(slots=4, backend=be)
+    out = srv.run(reqs, keep=False)
     summ = serve.summary(out)
-    summ.update({"precision": precision, "rate_per_s": rate, "mean_service_s": service,
-                 "calibration": f"first {n_cal} requests of the same seeded stream, one at a time",
-                 "slots": 4})
+    summ["loop_host_s"] = {k: round(v, 3) for k, v in srv.stats.items()}
+    target = SERVE_P99_TARGET_S[precision]
+    summ.update({"precision": precision, "rate_per_s": rate, "rate": "frozen: 0.7 / E[service]",
+                 "slots": 4, "cache_trims": be.trims, "p99_target_s": target,
+                 "p99_pass": bool(summ["p99_s"] <= target and summ["completed"] == n_req)})
     return summ
 
 
@@ -334,6 +412,10 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=512,
                     help="requests of the served stream (0 = skip it)")
+    ap.add_argument("--serve-modes", default="mixed,fp64",
+                    help="precisions the served stream runs in, comma-separated")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run the served stream alone and exit")
     ap.add_argument("--source-major", action="store_true",
                     help="time the device source-major copy of interaction-shaped matrices and exit")
     ap.add_argument("--c4", action="store_true",
@@ -351,6 +433,10 @@ def main() -> int:
               f"receiver rows with no exchange, and no multi-GPU figure is produced here.",
               file=sys.stderr)
         return 2
+    if args.serve_only:
+        print(json.dumps({"served": {m: served_stream(args.serve_requests, m)
+                                     for m in args.serve_modes.split(",") if m}}), flush=True)
+        return 0
     if args.counter_pass:
         counter_pass(args.mode)
         return 0
@@ -439,11 +525,13 @@ def main() -> int:
         "flop": {"fp64_rule": fl, "mixed_rule": flm},
     }
     if args.serve_requests > 0:
-        line["served"] = served_stream(args.serve_requests, args.mode)
+        from mhs_amd import serve
+        line["served"] = {m: served_stream(args.serve_requests, m)
+                          for m in args.serve_modes.split(",") if m}
+        serve.release_arenas()
     if not args.no_cpu:
-        cores = min(os.cpu_count() or 1, 16)
-        _progress(f"CPU baseline on {cores} processes")
-        line["cpu_baseline"] = cpu_baseline(args.cpu_cols, cores)
+        _progress(f"CPU baseline on {CPU_SHARE} processes")
+        line["cpu_baseline"] = cpu_baseline(args.cpu_cols)
         line["vs_cpu"] = value / line["cpu_baseline"]["value"]
     print(json.dumps(line), flush=True)
     return 0

@@ -229,6 +229,41 @@ class Device:
         a = np.ascontiguousarray(a, dtype=dtype or np.float64)
         return self.torch.from_numpy(a).to(self.dev)
 
+    def upload(self, arrays, stream):
+        """Host arrays to the device in one transfer on ``stream``: packed at
+        256-byte offsets into one pinned buffer (from torch's pinned cache,
+        which keeps it until the copy has completed), one asynchronous copy
+        into one device buffer; returns device views in order (None for
+        None).  The host does not wait for the copy."""
+        torch = self.torch
+        items, off = [], 0
+        for a in arrays:
+            if a is None:
+                items.append(None)
+                continue
+            a = np.ascontiguousarray(a)
+            items.append((off, a))
+            off += -(-a.nbytes // 256) * 256
+        total = max(off, 256)
+        pinned = torch.empty(total, dtype=torch.uint8, pin_memory=True)
+        buf = pinned.numpy()
+        for it in items:
+            if it is not None:
+                o, a = it
+                buf[o:o + a.nbytes] = a.reshape(-1).view(np.uint8)
+        with torch.cuda.stream(stream):
+            dbuf = torch.empty(total, dtype=torch.uint8, device=self.dev)
+            dbuf.copy_(pinned, non_blocking=True)
+        out = []
+        for it in items:
+            if it is None:
+                out.append(None)
+                continue
+            o, a = it
+            dt = torch.from_numpy(np.empty(0, a.dtype)).dtype
+            out.append(dbuf[o:o + a.nbytes].view(dt).view(a.shape))
+        return out
+
 
 _DEVICE = None
 
@@ -295,34 +330,62 @@ def _csr(ids: np.ndarray, n_nodes: int):
 class Prepared:
     """A job with its inputs resident in HBM and its outputs allocated: the
     host-side planning happens here, once; :meth:`launch` only enqueues
-    kernels (what a benchmark times and what the serving engine replays)."""
+    kernels (what a benchmark times and what the serving engine replays).
 
-    def __init__(self, job: Job, stream=None, dev: Device | None = None):
+    The inputs go to the device in one asynchronous copy (Device.upload).
+    ``alloc(nbytes)``, if given, supplies the large buffers (outputs, the
+    continuous layout's element scratch, the fold list and work) as uint8
+    device tensors (the serving loop carves them out of its arena); by
+    default they come from the tensor cache."""
+
+    def __init__(self, job: Job, stream=None, dev: Device | None = None, alloc=None):
         dev = dev or device()
         self.dev, self.job = dev, job
         torch = dev.torch
         self.stream = stream if stream is not None else torch.cuda.current_stream(dev.dev)
         self.handle = C.c_void_p(self.stream.cuda_stream)
+
+        def empty(shape, dtype=torch.float64):
+            if alloc is None:
+                return torch.empty(shape, dtype=dtype, device=dev.dev)
+            n = int(np.prod(shape)) * torch.empty(0, dtype=dtype).element_size()
+            return alloc(n).view(dtype).view(shape)
+
         K = job.K
         self.n_cols = n_cols = job.n_src * K
-        recs = geometry.source_records(job.tris, job.eps)
+        recs = job_records(job)
         strad = (np.flatnonzero(recs[:, geometry.S_STRADDLE] != 0.0).astype(np.int64)
                  if job.fold else np.zeros(0, np.int64))
         self.n_strad = len(strad)
         self.cont = cont = job.ids is not None
+        self.kind = _lib.KIND[job.kind]
+        self.flags = ((_lib.F_FOLD if job.fold else 0) | (_lib.F_SUBTRACT if job.subtract else 0)
+                      | (_lib.F_PARTS if job.parts else 0) | (_lib.F_MIXED if job.mixed else 0))
+        self.n_b_launch = 3 if (cont and job.kind == "interaction") else job.n_b
+        # the continuous layout's element columns take Cartesian slip
+        # (resolved on the node frames by the gather): unit source rows
+        sb = (np.broadcast_to(np.eye(3), (job.n_src, 3, 3)) if cont and job.kind == "interaction"
+              else None if cont else job.sb)
+        # the interaction runs the contracted passes (pointwise far field,
+        # closed-form near field, in output groups of one or two components),
+        # whose far pass has a tunable launch shape: column tiles per block
+        self.contracted = use_contracted(job.kind, sb is not None, job.n_a, self.n_b_launch,
+                                         job.order, job.mixed)
+        f64 = lambda a: None if a is None else np.asarray(a, np.float64)
+        host = [f64(job.obs), recs, f64(job.nrm), f64(job.rb), f64(sb),
+                strad if strad.size else None,
+                near_tile_bounds(recs, K) if self.contracted else None,
+                np.zeros(2, np.int32), np.zeros(1, np.int32)]
+        if cont:
+            ptr, idx = _csr(job.ids, job.n_nodes)
+            host += [ptr, idx, f64(job.sb) if job.kind == "interaction" else None]
+        dev_in = dev.upload(host, self.stream)
+        (self.obs_t, self.src_t, self.nrm_t, self.rb_t, self.sb_t, self.strad_t, self.near_tiles_t,
+         self.fold_cnt, self.fold_err) = dev_in[:9]
+        self.coef_t = dev.coef(job.mu, job.lam)
         with torch.cuda.stream(self.stream):
-            self.obs_t = dev.put(job.obs)
-            self.src_t = dev.put(recs)
-            self.nrm_t = dev.put(job.nrm)
-            self.rb_t = dev.put(job.rb)
-            # the continuous layout's element columns take Cartesian slip
-            # (resolved on the node frames by the gather): unit source rows
-            self.sb_t = (dev.put(np.broadcast_to(np.eye(3), (job.n_src, 3, 3))) if cont and job.kind == "interaction"
-                         else None if cont else dev.put(job.sb))
-            self.strad_t = dev.put(strad, np.int64) if strad.size else None
-            self.coef_t = dev.coef(job.mu, job.lam)
-            self.final = torch.empty(job.out_shape(), dtype=torch.float64, device=dev.dev)
-            self.final2 = torch.empty_like(self.final) if job.split else None
+            self.final = empty(job.out_shape())
+            self.final2 = empty(job.out_shape()) if job.split else None
             if cont:
                 elem_shape = list(job.out_shape(n_cols))
                 if job.kind == "interaction":
@@ -330,37 +393,19 @@ class Prepared:
                 self.elem_shape = elem_shape
                 row_bytes = 8 * int(np.prod(elem_shape[1:])) * (2 if job.split else 1)
                 self.rows = max(1, min(job.n_obs, ROW_SCRATCH_BYTES // max(row_bytes, 1)))
-                ptr, idx = _csr(job.ids, job.n_nodes)
-                self.ptr_t, self.idx_t = dev.put(ptr, np.int64), dev.put(idx, np.int64)
-                self.sbn_t = dev.put(job.sb) if job.kind == "interaction" else None
+                self.ptr_t, self.idx_t, self.sbn_t = dev_in[9:12]
                 scr_shape = [min(self.rows, job.n_obs)] + elem_shape[1:]
-                self.scr = torch.empty(scr_shape, dtype=torch.float64, device=dev.dev)
-                self.scr2 = torch.empty_like(self.scr) if job.split else None
+                self.scr = empty(scr_shape)
+                self.scr2 = empty(scr_shape) if job.split else None
             else:
                 self.rows = job.n_obs
             self.fold_cap = max(1, self.rows * self.n_strad)
-            self.fold_list = (torch.empty(2 * self.fold_cap, dtype=torch.int64, device=dev.dev)
-                              if strad.size else None)
+            self.fold_list = empty((2 * self.fold_cap,), torch.int64) if strad.size else None
             # the folded pairs' grids and centre loads: sized by the exact host
             # count of band pairs (any one launch holds at most all of them)
-            self.fold_items = fold_pairs_bound(job.obs, recs, strad) if strad.size else 0
-            self.fold_work = (torch.empty(self.fold_items * dev.lib.mhs_fold_item_bytes(job.order),
-                                          dtype=torch.uint8, device=dev.dev) if self.fold_items else None)
-            self.fold_cnt = torch.zeros(2, dtype=torch.int32, device=dev.dev)
-            self.fold_err = torch.zeros(1, dtype=torch.int32, device=dev.dev)
-        self.kind = _lib.KIND[job.kind]
-        self.flags = ((_lib.F_FOLD if job.fold else 0) | (_lib.F_SUBTRACT if job.subtract else 0)
-                      | (_lib.F_PARTS if job.parts else 0) | (_lib.F_MIXED if job.mixed else 0))
-        self.n_b_launch = 3 if (cont and job.kind == "interaction") else job.n_b
-        # the interaction runs the contracted passes (pointwise far field,
-        # closed-form near field, in output groups of one or two components),
-        # whose far pass has a tunable launch shape: column tiles per block
-        self.contracted = use_contracted(job.kind, self.sb_t is not None, job.n_a, self.n_b_launch,
-                                         job.order, job.mixed)
-        self.near_tiles_t = None
-        if self.contracted:
-            with torch.cuda.stream(self.stream):
-                self.near_tiles_t = dev.put(near_tile_bounds(recs, K))
+            self.fold_items = job_fold_bound(job, recs, strad)
+            self.fold_work = (empty((self.fold_items * dev.lib.mhs_fold_item_bytes(job.order),), torch.uint8)
+                              if self.fold_items else None)
         self.far_tiles = 0
         self.tiles_fixed = False
         self.rule = rule_struct(mixed_rule)
@@ -658,6 +703,26 @@ def fold_pairs_bound(obs: np.ndarray, recs: np.ndarray, strad: np.ndarray, chunk
     return n + 64
 
 
+def job_records(job: Job) -> np.ndarray:
+    """The job's source records (geometry.source_records), computed once and
+    kept on the job for the arrays they were computed from."""
+    hit = job.extra.get("records")
+    if hit is None or hit[0] is not job.tris or hit[1] is not job.eps:
+        hit = job.extra["records"] = (job.tris, job.eps, geometry.source_records(job.tris, job.eps))
+    return hit[2]
+
+
+def job_fold_bound(job: Job, recs: np.ndarray, strad: np.ndarray) -> int:
+    """fold_pairs_bound of the job, computed once and kept on the job for
+    the observer and source arrays it was computed from."""
+    if strad.size == 0:
+        return 0
+    hit = job.extra.get("fold_bound")
+    if hit is None or hit[0] is not job.obs or hit[1] is not job.tris or hit[2] is not job.eps:
+        hit = job.extra["fold_bound"] = (job.obs, job.tris, job.eps, fold_pairs_bound(job.obs, recs, strad))
+    return hit[3]
+
+
 def job_bytes(job: Job) -> int:
     """Device bytes a :class:`Prepared` of ``job`` holds: outputs, the
     continuous layout's element scratch, the fold list and the inputs."""
@@ -675,9 +740,9 @@ def job_bytes(job: Job) -> int:
     n_strad = int(np.sum(np.asarray(job.tris)[:, :, 2].max(axis=1) > -3.0 * np.asarray(job.eps)))
     fold = 16 * max(1, rows * n_strad) if job.fold and n_strad else 0
     if fold:
-        recs = geometry.source_records(job.tris, job.eps)
+        recs = job_records(job)
         strad = np.flatnonzero(recs[:, geometry.S_STRADDLE] != 0.0)
-        fold += FOLD_ITEM_BYTES * fold_pairs_bound(job.obs, recs, strad)
+        fold += FOLD_ITEM_BYTES * job_fold_bound(job, recs, strad)
     inputs = 8 * (job.n_obs * 3 * (2 + job.n_a) + job.n_src * (geometry.SRC_STRIDE + 3 * max(job.n_b, 0)))
     return int(out + scratch + fold + inputs)
 

@@ -7,8 +7,10 @@ precision.  The loop:
 * **admits** requests first-come first-served while a slot (a HIP stream of
   its own) is free and the request's predicted device bytes (outputs,
   scratch, fold list, inputs: :func:`runtime.job_bytes`) fit in what is left
-  of the HBM budget; a request larger than the whole budget is refused with
-  the same ``MemoryError`` arithmetic as the byte ceiling;
+  of the HBM budget; a request that does not fit yet is passed by later ones
+  that do for a bounded time (``bypass_s``), after which the queue waits for
+  it; a request larger than the whole budget is refused with the same
+  ``MemoryError`` arithmetic as the byte ceiling;
 * **interleaves** the admitted requests: each is cut into observer-row chunks
   of bounded pair count and the loop issues one chunk per active request in
   turn (at most ``depth`` in flight each), so a request that arrives while a
@@ -65,22 +67,139 @@ class Served:
 
 
 class _Active:
-    def __init__(self, index, prep, chunks, slot, nbytes):
-        self.index, self.prep, self.slot, self.bytes = index, prep, slot, nbytes
+    def __init__(self, index, prep, chunks, slot, nbytes, token):
+        self.index, self.prep, self.slot, self.bytes, self.token = index, prep, slot, nbytes, token
         self.pending = collections.deque(chunks)
         self.inflight = collections.deque()
 
 
-#: Device memory kept free of the tensor cache while serving: the HIP runtime
-#: allocates each hardware queue's scratch (the closed-form kernels hold up
-#: to 12.6 KB per lane, about 1.6 GB per queue at full occupancy) from what
-#: the cache has not reserved, and fails the queue when that runs out.
-SCRATCH_HEADROOM_BYTES = 16 << 30
+#: Device memory the tensor cache may never take while serving: the HIP
+#: runtime allocates each hardware queue's scratch (the closed-form kernels
+#: hold up to 12.6 KB per lane, gigabytes per queue at full occupancy) from
+#: what the cache has not reserved, and aborts the queue when that runs out
+#: (seen with 308 MB left: results of tens of GB had filled the cache).
+SCRATCH_HEADROOM_BYTES = 32 << 30
+#: Unused bytes of the tensor cache above which a completion returns them.
+TRIM_ABOVE_BYTES = 8 << 30
+#: Cached blocks at least this large (MiB) are neither split for smaller
+#: tensors nor kept when a larger request cannot be placed.
+MAX_SPLIT_MB = 1024
+
+
+class Arena:
+    """Byte ranges of one device buffer, first fit: requests of at least
+    ``small`` bytes from the low end, smaller ones from the high end (so
+    short-lived small results do not split the space large ones need);
+    freed ranges coalesce with their neighbours; ``align``-byte aligned.
+    Pure bookkeeping: :class:`DeviceArena` holds the memory."""
+
+    def __init__(self, nbytes: int, small: int = 1 << 30, align: int = 256):
+        self.align = int(align)
+        self.size = int(nbytes) - int(nbytes) % self.align
+        self.small = int(small)
+        self.free = [(0, self.size)] if self.size > 0 else []    # (offset, bytes), sorted, disjoint
+
+    def _round(self, n: int) -> int:
+        return max(self.align, -(-int(n) // self.align) * self.align)
+
+    def alloc(self, nbytes: int):
+        """Offset of a new range of ``nbytes`` (rounded up), or None."""
+        n = self._round(nbytes)
+        idx = range(len(self.free))
+        for k in (reversed(idx) if n < self.small else idx):
+            off, size = self.free[k]
+            if size < n:
+                continue
+            if n < self.small:             # from the high end of the block
+                self.free[k] = (off, size - n)
+                at = off + size - n
+            else:
+                self.free[k] = (off + n, size - n)
+                at = off
+            if self.free[k][1] == 0:
+                del self.free[k]
+            return at
+        return None
+
+    def release(self, off: int, nbytes: int) -> None:
+        n = self._round(nbytes)
+        k = 0
+        while k < len(self.free) and self.free[k][0] < off:
+            k += 1
+        if (k > 0 and self.free[k - 1][0] + self.free[k - 1][1] > off) or \
+                (k < len(self.free) and off + n > self.free[k][0]):
+            raise ValueError(f"range [{off}, {off + n}) is not allocated")
+        self.free.insert(k, (off, n))
+        if k + 1 < len(self.free) and off + n == self.free[k + 1][0]:
+            self.free[k] = (off, n + self.free[k + 1][1])
+            del self.free[k + 1]
+        if k > 0 and self.free[k - 1][0] + self.free[k - 1][1] == off:
+            self.free[k - 1] = (self.free[k - 1][0], self.free[k - 1][1] + self.free[k][1])
+            del self.free[k]
+
+    @property
+    def free_bytes(self) -> int:
+        return sum(s for _, s in self.free)
+
+    @property
+    def largest(self) -> int:
+        return max((s for _, s in self.free), default=0)
+
+
+class DeviceArena(Arena):
+    """An :class:`Arena` over one device buffer of the tensor cache: the
+    serving loop's results and scratch, allocated once (one hipMalloc
+    instead of one per request, and no device-wide synchronisation of a
+    free while other requests run)."""
+
+    def __init__(self, torch, dev, nbytes: int):
+        super().__init__(nbytes)
+        self.buf = torch.empty(self.size, dtype=torch.uint8, device=dev)
+
+    def bump(self, off: int, nbytes: int):
+        """An allocator handing out consecutive aligned slices of the range
+        [off, off + nbytes) as uint8 tensors (for runtime.Prepared)."""
+        end = off + nbytes
+        at = [off]
+
+        def take(n: int):
+            o = at[0]
+            at[0] = o + self._round(n)
+            if at[0] > end:
+                raise MemoryError(f"arena range of {nbytes} bytes exhausted")
+            return self.buf[o:o + n]
+        return take
+
+
+_ARENAS: dict = {}
+
+
+def device_arena(dev: runtime.Device, nbytes: int) -> DeviceArena:
+    """The device's serving arena, created at the first call with ``nbytes``
+    bytes and shared by every backend on the device after that."""
+    key = dev.dev.index
+    if key not in _ARENAS:
+        _ARENAS[key] = DeviceArena(dev.torch, dev.dev, nbytes)
+    return _ARENAS[key]
+
+
+def release_arenas() -> None:
+    """Drop the serving arenas (their memory returns to the tensor cache)."""
+    _ARENAS.clear()
 
 
 class GpuBackend:
     """Device side of the loop: one stream (default priority) per slot, the
     HIP kernels through :class:`runtime.Prepared`, completion by events.
+
+    The tensor cache is capped at all but ``headroom`` bytes of the device
+    (torch's per-process memory fraction: past it the cache first returns
+    its unused blocks, then raises), and after requests complete its unused
+    blocks are returned whenever they exceed ``trim_above`` bytes.  Both are
+    process-wide settings of the caching allocator, as is MAX_SPLIT_MB.
+    Requests whose results do not leave the loop take their device buffers
+    from the device's serving arena (:func:`device_arena`, the budget's
+    size, allocated at the first such request).
 
     With ``tune`` False (the default) a request uses the device's tuned
     mixed rule and far-pass tiles for its class if an earlier run tuned
@@ -91,26 +210,50 @@ class GpuBackend:
     def __init__(self, slots: int, dev: runtime.Device | None = None, headroom: int = SCRATCH_HEADROOM_BYTES,
                  tune: bool = False):
         self.dev = dev or runtime.device()
-        self.torch = self.dev.torch
-        self.streams = [self.torch.cuda.Stream(device=self.dev.dev) for _ in range(slots)]
-        self.total = int(self.torch.cuda.get_device_properties(self.dev.dev).total_memory)
+        torch = self.torch = self.dev.torch
+        self.total = int(torch.cuda.get_device_properties(self.dev.dev).total_memory)
         self.headroom = int(headroom)
+        if not 0 < self.headroom < self.total // 2:
+            raise ValueError(f"headroom must be in (0, {self.total // 2}) bytes, got {headroom}")
+        # results of tens of GB come and go: keep the tensor cache from
+        # carving small tensors out of a finished result's block (which pins
+        # the whole segment) and let it return such blocks when a large
+        # request does not fit
+        torch._C._accelerator_setAllocatorSettings(f"max_split_size_mb:{MAX_SPLIT_MB}")
+        torch.cuda.set_per_process_memory_fraction((self.total - self.headroom) / self.total, self.dev.dev)
+        self.trim_above = TRIM_ABOVE_BYTES
+        self.streams = [torch.cuda.Stream(device=self.dev.dev) for _ in range(slots)]
         self.tune = bool(tune)
         self.trims = 0          # times the cache was returned to the device
 
     def default_budget(self) -> int:
-        return int(0.8 * self.total)
+        """Bytes of admitted requests at once: 80 % of the device, within
+        the cache's cap (the serving arena's size)."""
+        return min(int(0.8 * self.total), self.total - self.headroom)
+
+    def reserve(self, nbytes: int):
+        """A range of the serving arena for a request's device buffers, or
+        None when no free range is large enough yet."""
+        arena = device_arena(self.dev, self.default_budget())
+        n = int(nbytes) + 4096
+        off = arena.alloc(n)
+        return None if off is None else (off, n)
+
+    def release(self, token) -> None:
+        device_arena(self.dev, self.default_budget()).release(*token)
 
     def trim(self) -> None:
-        """Return the tensor cache's unused blocks to the device when it has
-        reserved more than all but ``headroom`` bytes of it (requests of
-        many sizes fragment the cache until it holds nearly all of HBM)."""
-        if self.torch.cuda.memory_reserved(self.dev.dev) > self.total - self.headroom:
-            self.torch.cuda.empty_cache()
+        """Return the tensor cache's unused blocks to the device when they
+        exceed ``trim_above`` bytes (requests of many sizes leave it holding
+        blocks that fit no later request)."""
+        cuda = self.torch.cuda
+        if cuda.memory_reserved(self.dev.dev) - cuda.memory_allocated(self.dev.dev) > self.trim_above:
+            cuda.empty_cache()
             self.trims += 1
 
-    def prepare(self, job: runtime.Job, slot: int):
-        prep = runtime.Prepared(job, stream=self.streams[slot], dev=self.dev)
+    def prepare(self, job: runtime.Job, slot: int, token=None):
+        alloc = device_arena(self.dev, self.default_budget()).bump(*token) if token is not None else None
+        prep = runtime.Prepared(job, stream=self.streams[slot], dev=self.dev, alloc=alloc)
         if not self.tune:
             prep.fix_rule()
             prep.fix_tiles()
@@ -135,13 +278,19 @@ class Server:
     """The request loop.  ``slots`` concurrent requests, each on its own
     stream; ``budget_bytes`` of device memory for admitted work (default 80 %
     of the device); chunks of at most ``chunk_pairs`` pairs; ``depth`` chunks
-    in flight per request."""
+    in flight per request; a request that does not fit in the budget is
+    passed by later ones for at most ``bypass_s`` seconds after its arrival,
+    and after that by requests smaller than ``small_bytes`` only (default
+    1/64 of the budget; bypass_s = small_bytes = 0: strict first come,
+    first served)."""
 
     def __init__(self, slots: int = 4, budget_bytes: int | None = None, chunk_pairs: int = 1 << 25,
-                 depth: int = 2, backend=None):
+                 depth: int = 2, backend=None, bypass_s: float = 2.0, small_bytes: int | None = None):
         self.slots = int(slots)
+        self.bypass_s = float(bypass_s)
         self.backend = backend if backend is not None else GpuBackend(self.slots)
         self.budget = int(budget_bytes if budget_bytes is not None else self.backend.default_budget())
+        self.small_bytes = int(self.budget // 64 if small_bytes is None else small_bytes)
         self.chunk_pairs = int(chunk_pairs)
         self.depth = int(depth)
 
@@ -158,53 +307,86 @@ class Server:
         out2)``, if given, is called as each request completes (after its
         ``done`` time is taken) with its device results, and what it returns
         is kept in ``Served.out`` in place of the results, which are then
-        released: a consumer that copies or reduces each matrix as it lands."""
+        released: a consumer that copies or reduces each matrix as it lands
+        (it must have finished reading them when it returns).  Unless results
+        are kept (``keep`` without ``consume``), a backend with an arena
+        (``reserve``) places each request's device buffers in it."""
         be = self.backend
+        # device buffers from the backend's arena unless results leave the loop
+        arena = hasattr(be, "reserve") and (consume is not None or not keep)
         order = sorted(range(len(requests)), key=lambda i: requests[i].arrival)
         served = [Served(i, r.tag, float(r.arrival)) for i, r in enumerate(requests)]
-        waiting: collections.deque = collections.deque()
+        waiting: list = []
         active: list[_Active] = []
         free_slots = list(range(self.slots))
         in_use = 0
         nxt = 0
-        t0 = time.perf_counter()
+        # host seconds per step of the loop (the loop is one host thread:
+        # while it plans or prepares, nothing else is admitted or issued)
+        st = self.stats = dict.fromkeys(("plan", "prepare", "issue", "finish", "trim", "sleep"), 0.0)
+        clock = time.perf_counter
+        t0 = clock()
         while nxt < len(order) or waiting or active:
             now = time.perf_counter() - t0
             while nxt < len(order) and (not realtime or requests[order[nxt]].arrival <= now):
                 i = order[nxt]
                 nxt += 1
+                tp = clock()
                 try:
                     job = self._plan(requests[i])
                     waiting.append((i, job, runtime.job_bytes(job)))
                 except Exception as e:            # validation errors are the request's result
                     served[i].error = f"{type(e).__name__}: {e}"
                     served[i].done = served[i].admitted = time.perf_counter() - t0
-            # admission: first come, first served
-            while waiting and free_slots:
-                i, job, nbytes = waiting[0]
+                st["plan"] += clock() - tp
+            # admission: first come, first served, except that a request
+            # that does not fit in the budget yet lets later ones that fit
+            # pass it (first fit) until it has waited ``bypass_s``; then the
+            # queue holds for it while running requests drain, except for
+            # small requests (below ``small_bytes``), which still pass
+            k = 0
+            held = False
+            now = time.perf_counter() - t0
+            while free_slots and k < len(waiting):
+                i, job, nbytes = waiting[k]
                 if nbytes > self.budget:
-                    waiting.popleft()
+                    del waiting[k]
                     served[i].error = (f"MemoryError: needs {nbytes} device bytes, more than the "
                                        f"{self.budget}-byte serving budget")
                     served[i].done = served[i].admitted = time.perf_counter() - t0
                     continue
-                if in_use + nbytes > self.budget:
-                    break
-                waiting.popleft()
+                if held and nbytes >= self.small_bytes:
+                    k += 1
+                    continue
+                token = None
+                fits = in_use + nbytes <= self.budget
+                if fits and arena:
+                    token = be.reserve(nbytes)
+                    fits = token is not None
+                if not fits:
+                    if now - served[i].arrival >= self.bypass_s:
+                        held = True
+                    k += 1
+                    continue
+                del waiting[k]
                 slot = free_slots.pop(0)
-                prep = be.prepare(job, slot)
+                tp = clock()
+                prep = be.prepare(job, slot, token) if token is not None else be.prepare(job, slot)
+                st["prepare"] += clock() - tp
                 served[i].admitted = time.perf_counter() - t0
                 served[i].pairs = job.n_obs * job.n_src * job.K
                 served[i].bytes = nbytes
                 in_use += nbytes
-                active.append(_Active(i, prep, prep.chunks(self.chunk_pairs), slot, nbytes))
+                active.append(_Active(i, prep, prep.chunks(self.chunk_pairs), slot, nbytes, token))
             # issue: one chunk per active request in turn
             issued = False
+            tp = clock()
             for a in active:
                 while a.pending and len(a.inflight) < self.depth:
                     lo, hi = a.pending.popleft()
                     a.inflight.append(be.issue(a.prep, lo, hi))
                     issued = True
+            st["issue"] += clock() - tp
             # completions
             finished = []
             for a in active:
@@ -212,23 +394,32 @@ class Server:
                     a.inflight.popleft()
                 if not a.pending and not a.inflight:
                     finished.append(a)
+            tp = clock()
             for a in finished:
                 active.remove(a)
                 s = served[a.index]
                 s.done = time.perf_counter() - t0
                 try:
-                    out, out2 = be.finish(a.prep)
+                    res = be.finish(a.prep)
                     if consume is not None:
-                        s.out = consume(a.index, out, out2)
+                        s.out = consume(a.index, *res)
                     elif keep:
-                        s.out, s.out2 = out, out2
+                        s.out, s.out2 = res
+                    # no reference may outlive the request: its bytes are
+                    # released from the budget below
+                    del res
                 except RuntimeError as e:
                     s.error = str(e)
                 free_slots.append(a.slot)
                 in_use -= a.bytes
                 a.prep = None
+                if a.token is not None:
+                    be.release(a.token)
+            st["finish"] += clock() - tp
             if finished and hasattr(be, "trim"):
+                tp = clock()
                 be.trim()
+                st["trim"] += clock() - tp
             if not issued and not finished:
                 # nothing to do until a chunk completes or the next arrival:
                 # poll at a fine grain so that neither is noticed late
@@ -236,21 +427,29 @@ class Server:
                 if not active and nxt < len(order) and realtime:
                     wait = max(0.0, min(0.01, requests[order[nxt]].arrival - (time.perf_counter() - t0)))
                 if wait > 0:
+                    tp = clock()
                     time.sleep(wait)
+                    st["sleep"] += clock() - tp
         return served
 
 
 def summary(served: list[Served]) -> dict:
-    """p50 / p99 latency and pair-evals/s over the requests that completed."""
+    """p50 / p99 latency and pair-evals/s over the requests that completed,
+    with latency split into the wait for admission (admitted - arrival) and
+    the run (done - admitted)."""
     ok = [s for s in served if s.error is None]
     lat = np.array([s.latency for s in ok]) if ok else np.zeros(1)
+    wait = np.array([s.admitted - s.arrival for s in ok]) if ok else np.zeros(1)
+    run = lat - wait
     first = min(s.arrival for s in served) if served else 0.0
     last = max(s.done for s in ok) if ok else 0.0
     pairs = sum(s.pairs for s in ok)
     return {"requests": len(served), "completed": len(ok), "refused": len(served) - len(ok),
             "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
             "mean_s": float(lat.mean()), "makespan_s": float(last - first), "pairs": int(pairs),
-            "pair_evals_per_s": float(pairs / max(last - first, 1e-9))}
+            "pair_evals_per_s": float(pairs / max(last - first, 1e-9)),
+            "wait_p50_s": float(np.percentile(wait, 50)), "wait_p99_s": float(np.percentile(wait, 99)),
+            "run_p50_s": float(np.percentile(run, 50)), "run_p99_s": float(np.percentile(run, 99))}
 
 
 # ------------------------------------------------------------ request mix --

@@ -158,3 +158,128 @@ def test_cache_is_trimmed_as_requests_finish():
     done = serve.Server(slots=2, backend=be, chunk_pairs=1 << 14).run(reqs, keep=False, realtime=False)
     assert all(s.error is None for s in done)
     assert 1 <= be.trims <= len(reqs)
+
+
+def test_results_are_released_when_not_kept():
+    """keep=False: no reference to a finished request's result survives its
+    completion (its bytes are returned to the budget then)."""
+    import gc
+    import weakref
+
+    class Result:
+        pass
+
+    class Releasing(FakeBackend):
+        def __init__(self):
+            super().__init__()
+            self.refs = []
+
+        def finish(self, h):
+            super().finish(h)
+            gc.collect()
+            self.alive = max(getattr(self, "alive", 0), sum(r() is not None for r in self.refs))
+            r = Result()
+            self.refs.append(weakref.ref(r))
+            return r, None
+
+    be = Releasing()
+    reqs = [_disp_request(32, 0.0) for _ in range(4)]
+    done = serve.Server(slots=2, backend=be).run(reqs, keep=False, realtime=False)
+    gc.collect()
+    assert all(s.error is None and s.out is None for s in done)
+    assert len(be.refs) == 4 and all(r() is None for r in be.refs)
+    assert be.alive == 0          # none survived while the loop ran either
+
+
+@pytest.mark.parametrize("bypass_s,small", [(0.0, False), (60.0, False), (0.0, True)])
+def test_requests_that_fit_pass_a_blocked_one_for_a_while(bypass_s, small):
+    """First fit within ``bypass_s``: a request behind a large one that does
+    not fit yet is admitted past it; after that only requests below
+    ``small_bytes`` pass; with both 0 the queue is strictly first come,
+    first served."""
+    be = FakeBackend()
+    reqs = [_disp_request(200, 0.0), _disp_request(200, 0.0), _disp_request(10, 0.0)]
+    big = runtime.job_bytes(M.api.plan(reqs[0].fn, *reqs[0].args))
+    tail = runtime.job_bytes(M.api.plan(reqs[2].fn, *reqs[2].args))
+    srv = serve.Server(slots=4, budget_bytes=big + tail, backend=be, bypass_s=bypass_s,
+                       small_bytes=tail + 1 if small else 0)
+    out = srv.run(reqs, realtime=False)
+    assert all(s.error is None for s in out)
+    if bypass_s > 0 or small:
+        assert out[2].admitted < out[1].admitted
+    else:
+        assert out[2].admitted >= out[1].admitted
+
+
+def test_arena_ends_and_coalescing():
+    a = serve.Arena(10_000, small=1000, align=100)
+    big = a.alloc(2950)                 # rounds up to 3000, from the low end
+    small = a.alloc(150)                # 200, from the high end
+    mid = a.alloc(5000)
+    assert (big, small, mid) == (0, 9800, 3000)
+    assert a.free == [(8000, 1800)] and a.alloc(2000) is None
+    a.release(mid, 5000)
+    a.release(big, 2950)
+    assert a.free == [(0, 9800)]        # coalesced with both neighbours
+    with pytest.raises(ValueError):
+        a.release(0, 100)               # not allocated
+    a.release(small, 150)
+    assert a.free == [(0, 10_000)] and a.largest == 10_000
+
+
+def test_arena_random_alloc_release_keeps_invariants():
+    rng = np.random.default_rng(3)
+    a = serve.Arena(1 << 20, small=1 << 14, align=256)
+    live = []
+    for _ in range(2000):
+        if live and rng.uniform() < 0.45:
+            off, n = live.pop(int(rng.integers(len(live))))
+            a.release(off, n)
+        else:
+            n = int(np.exp(rng.uniform(np.log(10), np.log(1 << 17))))
+            off = a.alloc(n)
+            if off is not None:
+                live.append((off, n))
+        spans = sorted(a.free + [(o, a._round(n)) for o, n in live])
+        assert all(o1 + s1 == o2 for (o1, s1), (o2, _) in zip(spans, spans[1:]))   # a tiling
+        assert spans[0][0] == 0 and spans[-1][0] + spans[-1][1] == a.size
+        assert all(o1 + s1 < o2 for (o1, s1), (o2, _) in zip(a.free, a.free[1:]))  # coalesced
+    for off, n in live:
+        a.release(off, n)
+    assert a.free == [(0, a.size)]
+
+
+def test_server_places_requests_in_the_arena():
+    """A backend with an arena: each admitted request holds a range until it
+    completes; ranges of concurrent requests never overlap; all are returned."""
+    class ArenaBackend(FakeBackend):
+        def __init__(self, size):
+            super().__init__()
+            self.arena = serve.Arena(size, small=1 << 16)
+            self.held = {}
+
+        def reserve(self, nbytes):
+            off = self.arena.alloc(nbytes)
+            return None if off is None else (off, nbytes)
+
+        def release(self, token):
+            self.arena.release(*token)
+
+        def prepare(self, job, slot, token=None):
+            assert token is not None
+            lo, hi = token[0], token[0] + self.arena._round(token[1])
+            assert all(hi <= o or lo >= e for o, e in self.held.values())
+            h = super().prepare(job, slot)
+            self.held[h.serial] = (lo, hi)
+            return h
+
+        def finish(self, h):
+            del self.held[h.serial]
+            return super().finish(h)
+
+    reqs = [_disp_request(16 + 40 * (i % 4), 0.0) for i in range(12)]
+    size = 3 * runtime.job_bytes(M.api.plan(reqs[3].fn, *reqs[3].args))
+    be = ArenaBackend(size)
+    out = serve.Server(slots=4, backend=be, budget_bytes=size).run(reqs, keep=False, realtime=False)
+    assert all(s.error is None for s in out)
+    assert be.arena.free == [(0, be.arena.size)] and not be.held
