@@ -13,8 +13,10 @@ result is measured on all 10^8 entries.  Per-pass kernel times come from HIP
 events recorded on the launch stream; the roofline of the dominant kernel uses
 the kernels' own branch histogram and the static FLOP model of
 mhs_amd/flops.py.  The CPU baseline is the FP64 oracle (a numpy restatement
-of the reference) on every receiver against a seeded sample of source
-columns, the columns split over this host's cores.  Prints ONE JSON line.
+of the reference) on every receiver against a stratified seeded sample of
+source columns on the host's CPU share (16 processes), extrapolated to the
+whole matrix per stratum.  The served stream (512 requests at a frozen rate,
+both precisions) follows.  Prints ONE JSON line.
 """
 from __future__ import annotations
 
@@ -91,11 +93,13 @@ def cpu_baseline(n_cols: int) -> dict:
     source column at a time vectorised over the receivers) on all 10^4
     receivers x ``n_cols`` seeded source columns of the workload, one column
     per task on CPU_SHARE spawned processes (BLAS pinned to one thread),
-    surface-reaching (folded, costliest) columns first; ``value`` is the
-    sample's pairs over its wall time.  The sample is stratified (surface-
-    reaching sources, shallowest vertex above 3 eps depth, and the rest, in
-    proportion to their counts), each task's CPU seconds are kept per
-    stratum, and the whole 10^4 x 10^4 matrix is extrapolated from them.
+    surface-reaching (folded, costliest) columns first.  The sample is
+    stratified (surface-reaching sources, shallowest vertex above 3 eps
+    depth, and the rest, in proportion to their counts) and each task's CPU
+    seconds are kept per stratum; ``value`` is the whole 10^4 x 10^4 matrix
+    extrapolated from them, spread evenly over the processes (the sample's
+    own wall-clock rate, which two long folded columns dominate, is
+    reported beside it).
     Also: a one-process run (one column per stratum) and the restatement's
     serial us/pair on the published benchmark's size beside the published
     figure."""
@@ -122,11 +126,14 @@ def cpu_baseline(n_cols: int) -> dict:
     one, one_wall = _pool_run([("c3", np.array([cols[0]])), ("c3", np.array([cols[-1]]))], 1)
     pub, _ = _pool_run([("plane800", np.arange(20))], 1)
     phys = _physical_cores()
-    return {"value": pairs / wall, "unit": "pair-evals/s", "cores": procs, "kind": "port",
+    return {"value": n * n / (total_cpu / procs), "unit": "pair-evals/s", "cores": procs, "kind": "port",
             "sample": f"all 10,000 receivers x {n_cols} seeded source columns (seed {CPU_SEED}; {n_sh} of them "
                       f"surface-reaching, as in the whole mesh) of the same workload, FP64 oracle, "
                       f"{pairs:.3e} pairs (their fold pairs included) in {wall:.1f} s wall on {procs} "
-                      f"processes (the host's CPU share for one GPU)",
+                      f"processes (the host's CPU share for one GPU); value = the whole 10^4 x 10^4 "
+                      f"matrix extrapolated from each stratum's CPU seconds per pair, spread evenly "
+                      f"over the {procs} processes",
+            "sample_wall_pair_evals_per_s": pairs / wall,
             "host": {"os_cpu_count": os.cpu_count(), "lscpu_physical_cores": phys},
             "strata": per,
             "one_process": {"columns": 2, "pairs": sum(p for p, _ in one), "wall_s": one_wall,
@@ -322,12 +329,16 @@ def _progress(msg: str) -> None:
 #: one at a time on an MI355X and applied to all 512 (mixed 0.0807 s, FP64
 #: 0.175 s a request).
 SERVE_RATE = {"mixed": 8.68, "fp64": 4.00}
-#: The stated p99 latency targets.  The stream's size tail sets them: its
-#: largest request (an order-2 3 x 3 interaction of a 5,776-triangle mesh,
-#: 86 GB of output) takes 3.8 s mixed and 9.3 s FP64 alone; a processor-
-#: sharing simulation of the stream on the cost model gives p99 5.0 s and
-#: 13.7 s, and the targets leave 20 % above that.
-SERVE_P99_TARGET_S = {"mixed": 6.0, "fp64": 16.5}
+#: The stated latency targets.  The stream's size tail sets p99: its largest
+#: request (an order-2 3 x 3 interaction of a 7,482-triangle mesh, 146 GB of
+#: output) takes 3.8 s mixed and 9.2 s FP64 alone, and its four requests of
+#: 103-146 GB arrive within 10 s and cannot share the 230 GB arena, so they
+#: run one after another.  A processor-sharing simulation of the stream on
+#: the cost model with that memory limit and this admission policy gives p99
+#: 5.5 s mixed and 14.5 s FP64; the targets leave 20 % above that.  p50 is
+#: set by the many small requests: 0.25 s.
+SERVE_P99_TARGET_S = {"mixed": 6.6, "fp64": 17.4}
+SERVE_P50_TARGET_S = 0.25
 SERVE_WARM = 48      # requests served one at a time first (tuning the classes they visit)
 
 
@@ -354,7 +365,8 @@ def served_stream(n_req: int, precision: str) -> dict:
     target = SERVE_P99_TARGET_S[precision]
     summ.update({"precision": precision, "rate_per_s": rate, "rate": "frozen: 0.7 / E[service]",
                  "slots": 4, "cache_trims": be.trims, "p99_target_s": target,
-                 "p99_pass": bool(summ["p99_s"] <= target and summ["completed"] == n_req)})
+                 "p99_pass": bool(summ["p99_s"] <= target and summ["completed"] == n_req),
+                 "p50_target_s": SERVE_P50_TARGET_S, "p50_pass": bool(summ["p50_s"] <= SERVE_P50_TARGET_S)})
     return summ
 
 
