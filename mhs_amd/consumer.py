@@ -73,21 +73,67 @@ def _digest(v) -> object:
     return v
 
 
+#: interaction_matrix's keyword defaults (matrices.py:378-382): a keyword
+#: left out and the same keyword given at its default are one matrix.
+_DEFAULTS = {"receiver": ("strike", "dip", "normal"), "source": ("strike", "dip", "tensile"),
+             "order": 0, "shrink": api.COLLOCATION_SHRINK, "subtract_eigenstress": True,
+             "layout": "discontinuous", "fold": True}
+
+
+def _normalised(kwargs: dict) -> dict:
+    """The keywords with defaults filled in and equivalent spellings made
+    one: components as frame-row indices ('normal' is 'tensile', a string
+    is a one-element tuple), order as an int, booleans as bools."""
+    from . import tdcs
+    kw = dict(_DEFAULTS)
+    kw.update(kwargs)
+    for k in ("receiver", "source"):
+        kw[k] = [int(i) for i in tdcs.component_index(kw[k])]
+    kw["order"] = api._order(kw["order"])
+    kw["shrink"] = float(kw["shrink"])
+    for k in ("subtract_eigenstress", "fold"):
+        kw[k] = bool(kw[k])
+    return kw
+
+
+def kernel_tag() -> dict:
+    """What the device kernels do beyond the mesh and keywords, for the
+    cache key: the precision mode in force (a matrix built with FP32
+    far-field points is a different matrix from the FP64 one), the mixed
+    far-field rule fixed by the caller (None: tuned or the library default,
+    both bounded by the mixed mode's error bound), the near/far switch and
+    far-field Gauss orders, the image Q-family law and the fold's constants
+    (as studies/active_fractal/kernel.py:109-119 tags its fold)."""
+    from . import defaults as d
+    prec = runtime.get_precision()
+    return {
+        "precision": prec,
+        "mixed_rule": (repr(runtime.mixed_rule) if prec == "mixed" else None),
+        "near_far": [d.D_STAR, d.D_STAR_DISTANT, d.FAR_GAUSS_N, d.FAR_GAUSS_N_DISTANT],
+        "image_q": [d.IMAGE_Q_BUDGET_C, d.IMAGE_Q_GAUSS_MIN, d.IMAGE_Q_GAUSS_MAX],
+        "fold": [d.FOLD_SOURCE_DEPTH_EPS, d.FOLD_BAND_DEPTH_EPS, d.FOLD_REACH_EPS,
+                 d.FOLD_FINE_SPACING_EPS, d.FOLD_N_RADIAL, d.FOLD_N_ANGULAR],
+    }
+
+
 def cache_key(tris, material, eps, *, layout_tag: str = SOURCE_MAJOR, **kwargs) -> str:
     """A 16-hex-digit key of everything the interaction matrix depends on:
     the vertex array itself (SHA-256 of its float64 bytes, so a reordered
     or reversed mesh is a different key), the material, eps (scalar or
-    per-source), every keyword, the layout tag and this library's ABI
-    version (studies/active_fractal/kernel.py:131-155 keys on the same
-    things)."""
+    per-source), every keyword (defaults filled in, component names
+    normalised), the layout tag, the kernel tag (:func:`kernel_tag`: the
+    precision mode and rule in force, the thresholds) and this library's
+    ABI version (studies/active_fractal/kernel.py:131-155 keys on the same
+    things).  Call it under the precision the matrix is assembled in."""
     tris = np.ascontiguousarray(np.asarray(tris, float))
     payload = {
         "geometry_sha256": hashlib.sha256(tris.tobytes()).hexdigest(),
         "n": int(tris.reshape(-1, 3, 3).shape[0]),
         "mu": float(material.mu), "lam": float(material.lam),
         "eps": _digest(np.asarray(eps, float)) if np.ndim(eps) else float(eps),
-        "kwargs": {k: _digest(v) for k, v in sorted(kwargs.items())},
+        "kwargs": {k: _digest(v) for k, v in sorted(_normalised(kwargs).items())},
         "layout": layout_tag,
+        "kernel": kernel_tag(),
         "abi": _lib.ABI_VERSION,
     }
     blob = json.dumps(payload, sort_keys=True, default=str).encode()

@@ -30,6 +30,35 @@ def test_cache_key_is_stable_and_content_based():
     assert len(set(variants + [k])) == len(variants) + 1
 
 
+def test_cache_key_precision_rule_and_normalised_keywords():
+    """A matrix built in the mixed mode (FP32 far-field points) or under a
+    fixed mixed rule is not the FP64 matrix: its key differs.  Keywords at
+    their defaults, and 'normal' for 'tensile', are the same matrix: the
+    same key."""
+    from mhs_amd import runtime
+    tris, mat = _mesh(), M.Material(30.0, 30.0)
+    kw = dict(receiver="strike", source="strike")
+    k64 = consumer.cache_key(tris, mat, 0.1, **kw)
+    with M.precision("mixed"):
+        kmix = consumer.cache_key(tris, mat, 0.1, **kw)
+        old = runtime.mixed_rule
+        runtime.mixed_rule = runtime.RULE_SAFE
+        try:
+            krule = consumer.cache_key(tris, mat, 0.1, **kw)
+        finally:
+            runtime.mixed_rule = old
+    assert len({k64, kmix, krule}) == 3
+    assert k64 == consumer.cache_key(tris, mat, 0.1, **kw)
+    full = dict(receiver=("strike", "dip", "normal"), source=("strike", "dip", "tensile"), order=0,
+                layout="discontinuous", subtract_eigenstress=True, fold=True)
+    assert consumer.cache_key(tris, mat, 0.1) == consumer.cache_key(tris, mat, 0.1, **full)
+    assert consumer.cache_key(tris, mat, 0.1) == consumer.cache_key(
+        tris, mat, 0.1, receiver=("strike", "dip", "tensile"))
+    assert consumer.cache_key(tris, mat, 0.1, **kw) == consumer.cache_key(
+        tris, mat, 0.1, receiver=("strike",), source=["strike"])
+    assert consumer.cache_key(tris, mat, 0.1) != consumer.cache_key(tris, mat, 0.1, fold=False)
+
+
 def test_cache_key_per_source_eps_and_shared_nodes():
     tris, mat = _mesh(), M.Material(30.0, 30.0)
     e = np.linspace(0.05, 0.15, 12)
