@@ -11,7 +11,7 @@ import threading
 
 import numpy as np
 
-from . import chunking, runtime
+from . import chunking, defaults, runtime
 from .materials import Material
 from .shared import CONTINUOUS_ORDER, SharedNodes, check_layout, shared_nodes
 
@@ -119,6 +119,11 @@ def _finish(job, buf):
     if cap is not None:
         cap.append(job)
         return None
+    if buf is not None and not job.parts and 8 * buf.size > defaults.DEVICE_BLOCK_BYTES:
+        # matrices.py:169-189: out= may be larger than memory (a memmap);
+        # it is filled by receiver-row blocks of one device block each
+        from . import parallel
+        return parallel.stream_job(job, buf)
     out, out2 = runtime.run(job)
     res = runtime.to_host(out, buf)
     if job.parts:

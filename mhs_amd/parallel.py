@@ -222,6 +222,15 @@ def stream_rows(fn, *args, out=None, block_bytes: int | None = None, **kwargs):
         if out.dtype not in (np.float64, np.float32):
             raise ValueError(f"out must be float64 or float32, got {out.dtype}")
         buf = out
+    return stream_job(job, buf, block_bytes)
+
+
+def stream_job(job, buf, block_bytes: int | None = None):
+    """Assemble the validated ``job`` by receiver-row blocks into the host
+    array ``buf`` (its output shape, float64 or float32): the engine of
+    :func:`stream_rows`, also used by the public calls for an ``out=``
+    larger than one device block."""
+    shape = job.out_shape()
     n = job.n_obs
     row_elems = int(np.prod(shape[1:]))
     blocks = _device_blocks(n, 8 * row_elems, block_bytes)

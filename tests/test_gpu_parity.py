@@ -270,6 +270,40 @@ def test_stream_rows_equals_one_call(M, tmp_path):
         mp.stream_rows(M.stress_matrix, obs, tris, mat, 0.1, parts=True)
 
 
+def test_public_out_streams_beyond_one_block(M, tmp_path, monkeypatch):
+    """An out= larger than one device block (matrices.py:169-189: out= may
+    be a memmap larger than memory) is filled by receiver-row blocks, not
+    assembled whole on the device: bitwise the one-call matrix, into a
+    memmap and a float32 array."""
+    from mhs_amd import defaults, parallel as mp, workloads
+    rng = np.random.default_rng(8)
+    tris = workloads.orient_up(workloads.grid_mesh(workloads.plane(10.0, 5.0, 6, 3, 60.0, -0.5)))
+    mat = M.Material(30.0, 30.0)
+    obs = np.column_stack([rng.uniform(-8, 8, 50), rng.uniform(-8, 8, 50), -rng.uniform(0, 5, 50)])
+    nrm = rng.normal(size=(50, 3))
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    want = M.traction_matrix(obs, nrm, tris, mat, 0.1, order=1)
+    wint = M.interaction_matrix(tris, mat, 0.1, order=1)
+    calls = []
+    real = mp.stream_job
+    monkeypatch.setattr(mp, "stream_job", lambda job, buf, bb=None: calls.append(job.n_obs) or real(job, buf, bb))
+    monkeypatch.setattr(defaults, "DEVICE_BLOCK_BYTES", want[0].nbytes * 7)
+    print("case public out= memmap, traction order 1, 8 blocks", flush=True)
+    mm = np.lib.format.open_memmap(tmp_path / "t.npy", mode="w+", dtype=np.float64, shape=want.shape)
+    M.traction_matrix(obs, nrm, tris, mat, 0.1, order=1, out=mm)
+    mm.flush()
+    assert np.array_equal(np.load(tmp_path / "t.npy"), want)
+    o32 = np.zeros(want.shape, np.float32)
+    M.traction_matrix(obs, nrm, tris, mat, 0.1, order=1, out=o32)
+    assert np.array_equal(o32, want.astype(np.float32))
+    print("case public out= interaction order 1", flush=True)
+    monkeypatch.setattr(defaults, "DEVICE_BLOCK_BYTES", wint[0].nbytes * 5)
+    oi = np.zeros_like(wint)
+    M.interaction_matrix(tris, mat, 0.1, order=1, out=oi)
+    assert np.array_equal(oi, wint)
+    assert calls == [50, 50, wint.shape[0]]
+
+
 def test_source_major_layout(M):
     """The consumer layout (n_src, n_b, n_a, n_rec) copied on the device is
     bitwise the transpose of interaction_matrix, for 1, 2, 6 and 9
