@@ -127,6 +127,28 @@ typedef struct mhs_launch {
  * shared columns are summed afterwards by mhs_gather. */
 int mhs_assemble(const mhs_launch* launch, void* stream);
 
+/* Several assembly launches at once (the serving loop's batches): replaces
+ * the reference's pool of worker processes that each assemble one block of
+ * a matrix (src/mhs/parallel.py:132-221, by_source / by_rows), here for
+ * blocks of different matrices of uneven size.  Every tiled pass of every
+ * launch goes into a device work queue of (launch, observer-row tile,
+ * column or source tile) items, one queue per kernel instance, and each
+ * queue is drained by ONE persistent launch sized to the resident
+ * workgroups (store passes first, then add passes; the uncontracted
+ * Q-family pass keeps an ordinary launch per range: compiled with its
+ * parameter block in memory its point loop spills); then each launch's
+ * fold.
+ * The results are bitwise those of mhs_assemble on each launch.  The
+ * caller supplies two 256-byte aligned workspaces of at least
+ * mhs_batch_workspace_bytes(launches, n): host_ws pinned host memory, which
+ * the call fills and copies to dev_ws on the stream (both must stay
+ * untouched until the stream has passed the call), dev_ws device memory.
+ * No allocation; capturable in a HIP graph (the counters are cleared by the
+ * copy, not by a memset). */
+int64_t mhs_batch_workspace_bytes(const mhs_launch* launches, int n);
+int mhs_assemble_batch(const mhs_launch* launches, int n, void* dev_ws, void* host_ws,
+                       int64_t ws_bytes, void* stream);
+
 /* mhs_assemble for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
  * It has no counterpart in the reference.  The fold's work buffers come from

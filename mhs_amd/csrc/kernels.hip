@@ -24,6 +24,9 @@
 #include <string.h>
 
 #include <mutex>
+#include <vector>
+
+#define MHS_MAX_DEVICES 64   // devices a process may use the library on
 
 #include "fold.hpp"
 #include "../../include/mhs_amd.h"
@@ -105,6 +108,81 @@ static unsigned grid_1d(int64_t n) {
   return (unsigned)(b < 1 ? 1 : (b > (1 << 20) ? (1 << 20) : b));
 }
 
+// ------------------------------------------------------------ work queue --
+// A batch of assembly launches (several requests, meshes of uneven size)
+// runs each tiled pass as ONE persistent launch per kernel instance: the
+// tiles of every launch that needs that instance form a ragged queue of
+// segments, segment g holding the nbx x rows tiles of one launch unit
+// (parameter block p) from global tile index start on, in admission order.
+// Each workgroup takes the next tile index with one atomic, finds its
+// segment by binary search and runs the ordinary per-tile body on it, until
+// the queue is empty: no workgroup ever waits on another, and every
+// workgroup exits once the counter passes the total.
+static_assert(sizeof(Params) % 4 == 0, "parameter blocks are copied as 32-bit words");
+
+struct QueueSeg {
+  int64_t start;   // first global tile index of the segment
+  int32_t p;       // parameter block
+  int32_t nbx;     // tiles per row (the grid's x extent)
+};
+struct QueueArgs {
+  const Params* __restrict__ params;
+  const QueueSeg* __restrict__ seg;
+  unsigned long long* counter;       // zero before the launch
+  int64_t total;                     // tiles over all segments
+  int32_t n_seg;
+};
+
+// Segment of global tile b: (parameter block, bx, by).
+__device__ __forceinline__ void queue_tile(const QueueArgs& q, long long b, int& p, int& bx, int& by) {
+  int lo = 0, hi = q.n_seg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (q.seg[mid].start <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const long long r = b - q.seg[lo].start;
+  p = q.seg[lo].p;
+  bx = (int)(r % q.seg[lo].nbx);
+  by = (int)(r / q.seg[lo].nbx);
+}
+
+// Parameter block p, referenced in the constant address space as a kernel
+// argument is: its fields are scalar loads (the address is uniform and the
+// memory invariant during the launch), so the passes see them as uniform
+// values, and non-inlined calls receive a pointer to global memory, never to
+// a private copy.  (A private copy passed to such a call lived in scratch and
+// its fields came back as vector loads: uniform loop exits became divergent,
+// and the queue loop's compiled form then reloaded a register across its
+// back edge from a spill slot that a block run with no active lane had not
+// written -- a wild atomic address on the next tile.)
+__device__ __forceinline__ const Params& queue_params(const QueueArgs& q, int p) {
+  typedef const __attribute__((address_space(4))) Params* ConstParams;
+  const ConstParams c = (ConstParams)(q.params + p);
+  return *(const Params*)c;
+}
+
+template <class Body>
+__device__ __forceinline__ void drain_queue(const QueueArgs& q, Body body) {
+  __shared__ long long tile_s;
+  __shared__ int p_s, bx_s, by_s;
+  for (;;) {
+    __syncthreads();                 // the previous tile's LDS is no longer read
+    if (threadIdx.x == 0) {
+      const long long b = (long long)atomicAdd(q.counter, 1ULL);
+      tile_s = b;
+      if (b < q.total) queue_tile(q, b, p_s, bx_s, by_s);
+    }
+    __syncthreads();
+    if (tile_s >= q.total) return;   // the same in every lane
+    const int p = __builtin_amdgcn_readfirstlane(p_s);
+    const int64_t bx = __builtin_amdgcn_readfirstlane(bx_s), by = __builtin_amdgcn_readfirstlane(by_s);
+    body(queue_params(q, p), bx, by);
+  }
+}
+
+
+
 // ------------------------------------------------------------------ pair --
 // Row tiles of the pair kernels: a block (256 threads) takes one receiver
 // row and MHS_ROW_TILE items of it (columns, or sources) from c0 on, of n in
@@ -172,17 +250,16 @@ __device__ __attribute__((noinline)) void pair_item_call(const Params& pr, int64
 // and the higher occupancy were 5-10 % slower for it, and 10-25 % faster
 // for displacement.
 template <int P, int MODE>   // MODE 0: stress family (eigenstress included), 1: displacement
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == 1 ? MHS_PAIR_WAVES : 1)))
-k_pair(Params pr) {
+__device__ __forceinline__ void body_pair(const Params& pr, int64_t bx_, int64_t by_) {
   constexpr int K = (P + 1) * (P + 2) / 2;
   if (MODE != 1) {
-    const int64_t col = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (col < pr.n_cols) pair_item<P, MODE>(pr, pr.obs_lo + blockIdx.y, col);
+    const int64_t col = (int64_t)bx_ * 256 + threadIdx.x;
+    if (col < pr.n_cols) pair_item<P, MODE>(pr, pr.obs_lo + by_, col);
     return;
   }
   __shared__ int list_s[2 * MHS_ROW_TILE];
-  const int64_t o = pr.obs_lo + blockIdx.y;
-  const int64_t c0 = (int64_t)blockIdx.x * MHS_ROW_TILE;
+  const int64_t o = pr.obs_lo + by_;
+  const int64_t c0 = (int64_t)bx_ * MHS_ROW_TILE;
   const double ob[3] = {pr.obs[3 * o], pr.obs[3 * o + 1], pr.obs[3 * o + 2]};
   int n_far, n_near;
   list_row_tile(pr.n_cols, c0, [&](int64_t col) {
@@ -194,6 +271,15 @@ k_pair(Params pr) {
     pair_item_call<P, MODE>(pr, o, c0 + c);
   }
 }
+template <int P, int MODE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == 1 ? MHS_PAIR_WAVES : 1)))
+k_pair(Params pr) { body_pair<P, MODE>(pr, blockIdx.x, blockIdx.y); }
+// the same pass over the tiles of a batch's queue (see drain_queue)
+template <int P, int MODE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == 1 ? MHS_PAIR_WAVES : 1)))
+kq_pair(QueueArgs q) {
+  drain_queue(q, [&](const Params& pr, int64_t bx, int64_t by) { body_pair<P, MODE>(pr, bx, by); });
+}
 
 // --------------------------------------------------------------- Q-family --
 // Q-family of the uncontracted forms (disp, stress, traction, interaction
@@ -203,13 +289,13 @@ k_pair(Params pr) {
 // chunk's records are folded into the traction-operator tensor S
 // (image.py:107-162); the stress (or displacement) from S is added.
 template <int P, bool STRESS>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_qm(Params pr) {
+__device__ __forceinline__ void body_qm(const Params& pr, int64_t bx_, int64_t by_) {
   constexpr int K = (P + 1) * (P + 2) / 2;
   constexpr int NS = STRESS ? 27 : 9;
   constexpr int NCH = STRESS ? MHS_QMOM_DDG_CHUNKS : MHS_QMOM_DG_CHUNKS;
   constexpr int CH = STRESS ? MHS_QMOM_DDG_CHUNK : MHS_QMOM_DG_CHUNK;
-  const int64_t o = pr.obs_lo + blockIdx.y;
-  const int64_t col = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t o = pr.obs_lo + by_;
+  const int64_t col = (int64_t)bx_ * 256 + threadIdx.x;
   if (col >= pr.n_cols) return;
   const int64_t s = col / K;
   const int k = (int)(col - s * K);
@@ -277,6 +363,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     emit_disp(pr, o, col, S, true);
   }
 }
+template <int P, bool STRESS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_qm(Params pr) { body_qm<P, STRESS>(pr, blockIdx.x, blockIdx.y); }
+// (no queue form: compiled with its parameter block in memory instead of in
+// kernel arguments, its point loop spills about 10 KB per lane; a batch
+// launches it per range)
 
 
 // ------------------------------------------- contracted interaction, far --
@@ -436,9 +528,7 @@ __device__ __forceinline__ void far_pass(const Params& pr, const double* __restr
 }
 
 template <typename T, int P, int NOUT, int NPT, int LPP>
-__global__ void __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? MHS_FAR_WAVES32 : MHS_FAR_WAVES64)))
-k_far(Params pr) {
+__device__ __forceinline__ void body_far(const Params& pr, int64_t bx_, int64_t by_) {
   static_assert(LPP >= 4 && LPP <= 64 && (LPP & (LPP - 1)) == 0, "lanes per pair: a power of two in [4, 64]");
   constexpr int K = (P + 1) * (P + 2) / 2;
   constexpr int PPB = 256 / LPP;
@@ -452,7 +542,7 @@ k_far(Params pr) {
   __shared__ double res_s[PPB * K * NOUT];  // a tile's results, stored as one contiguous run
   const int tid = threadIdx.x;
   const int n_a = pr.n_a, n_b = pr.n_b;
-  const int64_t o = pr.obs_lo + blockIdx.y;
+  const int64_t o = pr.obs_lo + by_;
   const double ob[3] = {pr.obs[3 * o], pr.obs[3 * o + 1], pr.obs[3 * o + 2]};
   const double* nr = pr.nrm + 3 * o;
   // receiver half E_a (strain-space traction functional, frame row a)
@@ -490,7 +580,7 @@ k_far(Params pr) {
     lane &= 255u;
     const int grp = (int)(lane / LPP), li = (int)(lane % LPP);
     // a pair is (receiver row, source): its K node columns share every point
-    const int64_t s = ((int64_t)blockIdx.x * pr.far_tiles + tile) * PPB + grp;
+    const int64_t s = ((int64_t)bx_ * pr.far_tiles + tile) * PPB + grp;
     const bool valid = s < pr.n_src;
     int nd = 0, ni = 0, nq = 0;
     bool dfar = false, rfar = false, img = false;
@@ -630,7 +720,7 @@ k_far(Params pr) {
     __syncthreads();     // the kappas are rewritten by the next tile; res_s is complete
     {
       // the tile's PPB sources are PPB * K consecutive columns: one contiguous run of the row
-      const int64_t s0 = ((int64_t)blockIdx.x * pr.far_tiles + tile) * PPB;
+      const int64_t s0 = ((int64_t)bx_ * pr.far_tiles + tile) * PPB;
       const int64_t oo = o - pr.out_lo;
       const int64_t n_here = (pr.n_src - s0 < PPB ? pr.n_src - s0 : PPB) * K * NOUT;
       const int os = pr.na_tot * pr.nb_tot;   // entries per (row, column); more than NOUT in a group launch
@@ -642,6 +732,15 @@ k_far(Params pr) {
       }
     }
   }
+}
+template <typename T, int P, int NOUT, int NPT, int LPP>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? MHS_FAR_WAVES32 : MHS_FAR_WAVES64)))
+k_far(Params pr) { body_far<T, P, NOUT, NPT, LPP>(pr, blockIdx.x, blockIdx.y); }
+// the same pass over the tiles of a batch's queue (see drain_queue)
+template <typename T, int P, int NOUT, int NPT, int LPP>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? MHS_FAR_WAVES32 : MHS_FAR_WAVES64)))
+kq_far(QueueArgs q) {
+  drain_queue(q, [&](const Params& pr, int64_t bx, int64_t by) { body_far<T, P, NOUT, NPT, LPP>(pr, bx, by); });
 }
 
 // ------------------------------------------------------ mixed far pass --
@@ -791,8 +890,7 @@ __device__ __forceinline__ void mixed_family(const Params& pr, const double* __r
 }
 
 template <int P, int NOUT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MHS_FARM_WAVES)))
-k_far_mixed(Params pr) {
+__device__ __forceinline__ void body_far_mixed(const Params& pr, int64_t bx_, int64_t by_) {
   constexpr int K = (P + 1) * (P + 2) / 2;
   constexpr int KN = K * NOUT;
   __shared__ __align__(16) float BQ_s[2][MHS_QM * 8];
@@ -801,7 +899,7 @@ k_far_mixed(Params pr) {
   __shared__ double res_s[256 * KN];
   const int tid = threadIdx.x;
   const int n_a = pr.n_a, n_b = pr.n_b;
-  const int64_t o = pr.obs_lo + blockIdx.y;
+  const int64_t o = pr.obs_lo + by_;
   const double ob[3] = {pr.obs[3 * o], pr.obs[3 * o + 1], pr.obs[3 * o + 2]};
   const double* nr = pr.nrm + 3 * o;
   if (tid < n_a * 9) {
@@ -830,7 +928,7 @@ k_far_mixed(Params pr) {
   }
   __syncthreads();
   for (int tile = 0; tile < pr.far_tiles; ++tile) {
-    const int64_t s = ((int64_t)blockIdx.x * pr.far_tiles + tile) * 256 + tid;
+    const int64_t s = ((int64_t)bx_ * pr.far_tiles + tile) * 256 + tid;
     double acc[K][NOUT];
 #pragma unroll
     for (int k = 0; k < K; ++k)
@@ -935,7 +1033,7 @@ k_far_mixed(Params pr) {
         for (int t = 0; t < NOUT; ++t) res_s[(tid * K + k) * NOUT + t] = acc[k][t];
       __syncthreads();
       // the tile's sources are 256 K consecutive columns: one contiguous run of the row
-      const int64_t s0 = ((int64_t)blockIdx.x * pr.far_tiles + tile) * 256;
+      const int64_t s0 = ((int64_t)bx_ * pr.far_tiles + tile) * 256;
       const int64_t n_here = (pr.n_src - s0 < 256 ? pr.n_src - s0 : 256) * KN;
       double* dst = pr.out + (oo * pr.n_cols + s0 * K) * os;
       if (os == NOUT) {
@@ -946,6 +1044,15 @@ k_far_mixed(Params pr) {
       __syncthreads();
     }
   }
+}
+template <int P, int NOUT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MHS_FARM_WAVES)))
+k_far_mixed(Params pr) { body_far_mixed<P, NOUT>(pr, blockIdx.x, blockIdx.y); }
+// the same pass over the tiles of a batch's queue (see drain_queue)
+template <int P, int NOUT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MHS_FARM_WAVES)))
+kq_far_mixed(QueueArgs q) {
+  drain_queue(q, [&](const Params& pr, int64_t bx, int64_t by) { body_far_mixed<P, NOUT>(pr, bx, by); });
 }
 
 // Near parts of the contracted interaction (closed forms, folded mirror),
@@ -966,11 +1073,11 @@ __device__ __attribute__((noinline)) void near_item_call(const Params& pr, int64
 // was faster at order 2 while the image edge integrals ran as a memory-bound
 // loop, and slower once they were unrolled: 84 against 66 ms at n = 1560.)
 template <int P>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MHS_NEAR_WAVES))) k_near(Params pr) {
+__device__ __forceinline__ void body_near(const Params& pr, int64_t bx_, int64_t by_) {
   constexpr int K = (P + 1) * (P + 2) / 2;
   __shared__ int list_s[2 * MHS_ROW_TILE];
-  const int64_t o = pr.obs_lo + blockIdx.y;
-  const int64_t c0 = (int64_t)blockIdx.x * MHS_ROW_TILE;
+  const int64_t o = pr.obs_lo + by_;
+  const int64_t c0 = (int64_t)bx_ * MHS_ROW_TILE;
   const double ob[3] = {pr.obs[3 * o], pr.obs[3 * o + 1], pr.obs[3 * o + 2]};
   int n_img, n_dir;
   // a step of 256 columns whose sources' centroids and mirrored centroids
@@ -993,6 +1100,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MHS_NE
     const int c = i < n_img ? list_s[i] : list_s[MHS_ROW_TILE + i - n_img];
     near_item_call<P>(pr, o, c0 + c);
   }
+}
+template <int P>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MHS_NEAR_WAVES)))
+k_near(Params pr) { body_near<P>(pr, blockIdx.x, blockIdx.y); }
+// the same pass over the tiles of a batch's queue (see drain_queue)
+template <int P>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MHS_NEAR_WAVES)))
+kq_near(QueueArgs q) {
+  drain_queue(q, [&](const Params& pr, int64_t bx, int64_t by) { body_near<P>(pr, bx, by); });
 }
 
 __global__ void k_zero_i32(int* p, int n) {
@@ -1476,27 +1592,49 @@ static int fill_rule(const mhs_launch* L, Params& pr) {
 // packet) can overflow however large the matrix.
 #define MHS_ROWS_PER_LAUNCH 32768
 
+// The tiled passes of a launch, as a plan: each item is one kernel instance
+// (variant) over a 2-D grid (x: column or source tiles, y: observer rows of
+// a chunk of at most MHS_ROWS_PER_LAUNCH) with its parameter block.  Store
+// passes write every entry of their rows, add passes add; a launch's add
+// items follow all its store items.  launch_all runs the plan as ordinary
+// launches; mhs_assemble_batch merges the plans of several launches into one
+// persistent launch per variant and stage.
+enum Variant {
+  V_PAIR_S,      // k_pair<P, 0>: stress family, store
+  V_PAIR_D,      // k_pair<P, 1>: displacement, store
+  V_FAR64_1,     // k_far (FP64 points), one output, store
+  V_FAR64_2,     // k_far (FP64 points), two outputs, store
+  V_FARM_1,      // k_far_mixed, one output, store
+  V_FARM_2,      // k_far_mixed, two outputs, store
+  V_QM_S,        // k_qm<P, true>: stress family, add
+  V_QM_D,        // k_qm<P, false>: displacement, add
+  V_NEAR,        // k_near<P>: contracted near parts, add
+  V_COUNT
+};
+static inline bool is_store(int v) { return v <= V_FARM_2; }
+
+struct PlanItem {
+  int var;
+  Params pr;
+  int64_t gx, gy;
+};
+
 template <int P>
-static int launch_all(const mhs_launch* L, const Params& pr, hipStream_t st) {
+static int plan_passes(const mhs_launch* L, const Params& pr, std::vector<PlanItem>& plan) {
   const int64_t n_rows = L->obs_hi - L->obs_lo;
   if (n_rows <= 0 || pr.n_cols <= 0) return 0;
   const int64_t nblk = (pr.n_cols + 255) / 256;
   if (nblk * 256 > 0xffffffffLL) return fail("mhs_assemble: too many columns for one launch");
-  const dim3 block(256);
-  const bool stress = pr.kind != K_DISP;
   const int stages = L->stages ? L->stages : (MHS_STAGE_PAIR | MHS_STAGE_Q | MHS_STAGE_FOLD);
   const int nout = pr.n_a * pr.n_b;
   const bool contracted = use_contracted(pr.kind, pr.sb != nullptr, pr.n_a, pr.n_b, P, pr.flags);
   const bool mixed = pr.flags & F_MIXED;
-  // points per lane (two outputs: half as many)
-  constexpr int N64 = MHS_FAR_NPT64;
-  constexpr int N64b = N64 > 1 ? N64 / 2 : 1;
+  const int64_t col_tiles = (pr.n_cols + MHS_ROW_TILE - 1) / MHS_ROW_TILE;
+  std::vector<PlanItem> adds;
   for (int64_t lo = L->obs_lo; lo < L->obs_hi; lo += MHS_ROWS_PER_LAUNCH) {
     const int64_t rows = (L->obs_hi - lo) < MHS_ROWS_PER_LAUNCH ? (L->obs_hi - lo) : MHS_ROWS_PER_LAUNCH;
     Params pc = pr;
     pc.obs_lo = lo;
-    const dim3 grid((unsigned)nblk, (unsigned)rows);
-    const dim3 tiles((unsigned)((pr.n_cols + MHS_ROW_TILE - 1) / MHS_ROW_TILE), (unsigned)rows);   // column tiles
     if (contracted) {
       // output groups: the whole launch when it has one or two outputs, else
       // one receiver frame row a with source rows b0, b0 + 1 per group (the
@@ -1515,67 +1653,258 @@ static int launch_all(const mhs_launch* L, const Params& pr, hipStream_t st) {
           const int gout = pg.n_a * pg.n_b;
           // far pass (writes every entry of the group), then the near pass (adds)
           if (stages & MHS_STAGE_Q) {
-            auto fg = [&](int lpp) {   // sources per block = 256 / lanes per pair, per tile
-              const int64_t per = (int64_t)(256 / lpp) * pr.far_tiles;
-              return dim3((unsigned)((pr.n_src + per - 1) / per), (unsigned)rows);
-            };
-            constexpr int L64 = MHS_FAR_LPP64, L64b = MHS_FAR_LPP64B;
             // the mixed mode: a lane per pair at the mixed rule's orders
-            if (gout == 1 && mixed) hipLaunchKernelGGL((k_far_mixed<P, 1>), fg(1), block, 0, st, pg);
-            else if (gout == 1) hipLaunchKernelGGL((k_far<double, P, 1, N64, L64>), fg(L64), block, 0, st, pg);
-            else if (mixed) hipLaunchKernelGGL((k_far_mixed<P, 2>), fg(1), block, 0, st, pg);
-            else hipLaunchKernelGGL((k_far<double, P, 2, N64b, L64b>), fg(L64b), block, 0, st, pg);
-            if (check(hipGetLastError(), "k_far launch")) return -1;
+            const int lpp = mixed ? 1 : (gout == 1 ? MHS_FAR_LPP64 : MHS_FAR_LPP64B);
+            const int64_t per = (int64_t)(256 / lpp) * pr.far_tiles;   // sources per block
+            const int var = gout == 1 ? (mixed ? V_FARM_1 : V_FAR64_1) : (mixed ? V_FARM_2 : V_FAR64_2);
+            plan.push_back({var, pg, (pr.n_src + per - 1) / per, rows});
           }
-          if (stages & MHS_STAGE_PAIR) {
-            hipLaunchKernelGGL((k_near<P>), tiles, block, 0, st, pg);
-            if (check(hipGetLastError(), "k_near launch")) return -1;
-          }
+          if (stages & MHS_STAGE_PAIR) adds.push_back({V_NEAR, pg, col_tiles, rows});
         }
       continue;
     }
     if (stages & MHS_STAGE_PAIR) {
-      if (pr.kind == K_DISP) hipLaunchKernelGGL((k_pair<P, 1>), tiles, block, 0, st, pc);
-      else hipLaunchKernelGGL((k_pair<P, 0>), grid, block, 0, st, pc);
-      if (check(hipGetLastError(), "k_pair launch")) return -1;
+      if (pr.kind == K_DISP) plan.push_back({V_PAIR_D, pc, col_tiles, rows});
+      else plan.push_back({V_PAIR_S, pc, nblk, rows});
     }
-    if (pr.kind == K_EIGEN || !(stages & MHS_STAGE_Q)) continue;
-    {
-      if (stress) hipLaunchKernelGGL((k_qm<P, true>), grid, block, 0, st, pc);
-      else hipLaunchKernelGGL((k_qm<P, false>), grid, block, 0, st, pc);
-      if (check(hipGetLastError(), "k_qm launch")) return -1;
+    if (pr.kind != K_EIGEN && (stages & MHS_STAGE_Q)) adds.push_back({pr.kind != K_DISP ? V_QM_S : V_QM_D, pc, nblk, rows});
+  }
+  plan.insert(plan.end(), adds.begin(), adds.end());
+  return 0;
+}
+
+// One plan item as an ordinary launch.
+template <int P>
+static int launch_item(const PlanItem& it, hipStream_t st) {
+  const dim3 g((unsigned)it.gx, (unsigned)it.gy), b(256);
+  constexpr int N64 = MHS_FAR_NPT64;
+  constexpr int N64b = N64 > 1 ? N64 / 2 : 1;
+  constexpr int L64 = MHS_FAR_LPP64, L64b = MHS_FAR_LPP64B;
+  switch (it.var) {
+    case V_PAIR_S: hipLaunchKernelGGL((k_pair<P, 0>), g, b, 0, st, it.pr); break;
+    case V_PAIR_D: hipLaunchKernelGGL((k_pair<P, 1>), g, b, 0, st, it.pr); break;
+    case V_FAR64_1: hipLaunchKernelGGL((k_far<double, P, 1, N64, L64>), g, b, 0, st, it.pr); break;
+    case V_FAR64_2: hipLaunchKernelGGL((k_far<double, P, 2, N64b, L64b>), g, b, 0, st, it.pr); break;
+    case V_FARM_1: hipLaunchKernelGGL((k_far_mixed<P, 1>), g, b, 0, st, it.pr); break;
+    case V_FARM_2: hipLaunchKernelGGL((k_far_mixed<P, 2>), g, b, 0, st, it.pr); break;
+    case V_QM_S: hipLaunchKernelGGL((k_qm<P, true>), g, b, 0, st, it.pr); break;
+    case V_QM_D: hipLaunchKernelGGL((k_qm<P, false>), g, b, 0, st, it.pr); break;
+    default: hipLaunchKernelGGL((k_near<P>), g, b, 0, st, it.pr); break;
+  }
+  return check(hipGetLastError(), "mhs_assemble: pass launch");
+}
+
+// The persistent (queue) kernel of a variant and its resident workgroups per CU.
+template <int P>
+static const void* queue_kernel(int v) {
+  constexpr int N64 = MHS_FAR_NPT64;
+  constexpr int N64b = N64 > 1 ? N64 / 2 : 1;
+  constexpr int L64 = MHS_FAR_LPP64, L64b = MHS_FAR_LPP64B;
+  switch (v) {
+    case V_PAIR_S: return (const void*)&kq_pair<P, 0>;
+    case V_PAIR_D: return (const void*)&kq_pair<P, 1>;
+    case V_FAR64_1: return (const void*)&kq_far<double, P, 1, N64, L64>;
+    case V_FAR64_2: return (const void*)&kq_far<double, P, 2, N64b, L64b>;
+    case V_FARM_1: return (const void*)&kq_far_mixed<P, 1>;
+    case V_FARM_2: return (const void*)&kq_far_mixed<P, 2>;
+    case V_QM_S:
+    case V_QM_D: return nullptr;          // launched per range (see mhs_assemble_batch)
+    default: return (const void*)&kq_near<P>;
+  }
+}
+
+template <int P>
+static int launch_queue(int v, unsigned grid, const QueueArgs& q, hipStream_t st) {
+  const dim3 g(grid), b(256);
+  constexpr int N64 = MHS_FAR_NPT64;
+  constexpr int N64b = N64 > 1 ? N64 / 2 : 1;
+  constexpr int L64 = MHS_FAR_LPP64, L64b = MHS_FAR_LPP64B;
+  switch (v) {
+    case V_PAIR_S: hipLaunchKernelGGL((kq_pair<P, 0>), g, b, 0, st, q); break;
+    case V_PAIR_D: hipLaunchKernelGGL((kq_pair<P, 1>), g, b, 0, st, q); break;
+    case V_FAR64_1: hipLaunchKernelGGL((kq_far<double, P, 1, N64, L64>), g, b, 0, st, q); break;
+    case V_FAR64_2: hipLaunchKernelGGL((kq_far<double, P, 2, N64b, L64b>), g, b, 0, st, q); break;
+    case V_FARM_1: hipLaunchKernelGGL((kq_far_mixed<P, 1>), g, b, 0, st, q); break;
+    case V_FARM_2: hipLaunchKernelGGL((kq_far_mixed<P, 2>), g, b, 0, st, q); break;
+    case V_QM_S:
+    case V_QM_D: return fail("mhs_assemble_batch: the Q-family pass has no queue form");
+    default: hipLaunchKernelGGL((kq_near<P>), g, b, 0, st, q); break;
+  }
+  return check(hipGetLastError(), "mhs_assemble_batch: queue launch");
+}
+
+// The fold of a launch: pair list, grids and centre loads, near and far points.
+template <int P>
+static int launch_fold(const mhs_launch* L, const Params& pr, hipStream_t st) {
+  const int stages = L->stages ? L->stages : (MHS_STAGE_PAIR | MHS_STAGE_Q | MHS_STAGE_FOLD);
+  const int64_t n_rows = L->obs_hi - L->obs_lo;
+  if (pr.kind == K_EIGEN || n_rows <= 0 || pr.n_cols <= 0) return 0;
+  if (!((stages & MHS_STAGE_FOLD) && (pr.flags & F_FOLD) && L->n_strad > 0)) return 0;
+  const bool stress = pr.kind != K_DISP;
+  const dim3 block(256);
+  if (!L->fold_list || !L->fold_count || !L->fold_err) return fail("mhs_assemble: fold buffers missing");
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, L->fold_count, 2);
+  const int64_t nt = n_rows * L->n_strad;
+  hipLaunchKernelGGL(k_fold_list, dim3(grid_1d(nt)), dim3(256), 0, st, pr, L->strad,
+                     L->n_strad, n_rows, L->fold_list, L->fold_count, L->fold_cap);
+  if (check(hipGetLastError(), "k_fold_list launch")) return -1;
+  if (!L->fold_work || L->fold_work_items < 1) return fail("mhs_assemble: fold work buffer missing");
+  FoldItem* work = (FoldItem*)L->fold_work;
+  const int64_t prep_blocks = (L->fold_work_items + 63) / 64;
+  hipLaunchKernelGGL((k_fold_prep<P>), dim3((unsigned)(prep_blocks < 4096 ? prep_blocks : 4096)), dim3(64), 0, st,
+                     pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items, L->fold_err);
+  if (check(hipGetLastError(), "k_fold_prep launch")) return -1;
+  const dim3 fg(2048);
+  if (stress) {
+    hipLaunchKernelGGL((k_fold<P, true, true>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items);
+    if (pr.mixed_rule)
+      hipLaunchKernelGGL((k_fold<P, true, false, true>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items);
+    else
+      hipLaunchKernelGGL((k_fold<P, true, false>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items);
+  } else {
+    hipLaunchKernelGGL((k_fold<P, false, true>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items);
+    if (pr.mixed_rule)
+      hipLaunchKernelGGL((k_fold<P, false, false, true>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items);
+    else
+      hipLaunchKernelGGL((k_fold<P, false, false>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items);
+  }
+  return check(hipGetLastError(), "k_fold launch");
+}
+
+template <int P>
+static int launch_all(const mhs_launch* L, const Params& pr, hipStream_t st) {
+  std::vector<PlanItem> plan;
+  if (plan_passes<P>(L, pr, plan)) return -1;
+  for (const PlanItem& it : plan)
+    if (launch_item<P>(it, st)) return -1;
+  return launch_fold<P>(L, pr, st);
+}
+
+// Validated parameter block of one launch descriptor.
+static int make_params(const mhs_launch* L, Params& pr) {
+  if (!L) return fail("mhs_assemble: null launch descriptor");
+  if (L->abi_version != MHS_ABI_VERSION) return fail("mhs_assemble: ABI version mismatch");
+  if (L->order < 0 || L->order > 2) return fail("mhs_assemble: order must be 0, 1 or 2");
+  if (L->kind < K_DISP || L->kind > K_INTERACTION) return fail("mhs_assemble: unknown kind");
+  if (!L->obs || !L->src || !L->coef || !L->gl_x || !L->gl_w || !L->gq || !L->out)
+    return fail("mhs_assemble: null device pointer");
+  if ((L->kind == K_TRACTION || L->kind == K_INTERACTION) && !L->nrm)
+    return fail("mhs_assemble: traction/interaction need receiver normals");
+  if (L->kind == K_INTERACTION && (!L->rb || L->n_a < 1 || L->n_a > 3 || L->n_b < 1 || L->n_b > 3))
+    return fail("mhs_assemble: interaction needs 1..3 receiver and source components");
+  if (L->kind == K_INTERACTION && !L->sb && L->n_b != 3)
+    return fail("mhs_assemble: identity source basis needs n_b = 3");
+  if ((L->flags & F_PARTS) && L->kind != K_STRESS)
+    return fail("mhs_assemble: parts needs the stress kind");
+  if (L->kind == K_STRESS && (L->flags & (F_PARTS | F_SUBTRACT)) && !L->out2)
+    return fail("mhs_assemble: the stress kind with subtract or parts needs out2 for the eigenstress");
+  if (L->obs_lo < 0 || L->obs_hi > L->n_obs || L->obs_lo > L->obs_hi)
+    return fail("mhs_assemble: bad observer row range");
+  const int K = (L->order + 1) * (L->order + 2) / 2;
+  if (L->n_cols != L->n_src * K) return fail("mhs_assemble: n_cols must be n_src * K");
+  pr.obs = L->obs; pr.nrm = L->nrm; pr.rb = L->rb; pr.sb = L->sb; pr.src = L->src;
+  pr.coef = L->coef; pr.gl_x = L->gl_x; pr.gl_w = L->gl_w; pr.gq = L->gq; pr.out = L->out; pr.out2 = L->out2;
+  pr.n_obs = L->n_obs; pr.n_src = L->n_src; pr.n_cols = L->n_cols; pr.obs_lo = L->obs_lo;
+  pr.out_lo = L->obs_lo;
+  pr.near_tiles = L->near_tiles;
+  pr.kind = L->kind; pr.flags = L->flags; pr.n_a = L->n_a; pr.n_b = L->n_b;
+  pr.mu = L->mu; pr.lam = L->lam; pr.nu = 0.5 * L->lam / (L->lam + L->mu);
+  {
+    const double c34 = 3.0 - 4.0 * pr.nu;
+    pr.C1 = 1.0 / (16.0 * M_PI * pr.mu * (1.0 - pr.nu));
+    pr.k5e = -6.0 * (1.0 - pr.nu);
+    pr.k7c = 30.0 * (1.0 - pr.nu);
+    pr.k7e = 15.0 / (8.0 * M_PI * pr.C1);
+    pr.k3 = 1.0 - c34;
+    pr.k5c = 3.0 * c34 - 9.0;
+  }
+  if (L->far_tiles < 0 || L->far_tiles > 4096) return fail("mhs_assemble: far_tiles out of range");
+  const bool contracted = use_contracted(L->kind, L->sb != nullptr, L->n_a, L->n_b, L->order, L->flags);
+  pr.na_tot = L->n_a;
+  pr.nb_tot = L->n_b;
+  pr.far_tiles = L->far_tiles ? L->far_tiles : ((contracted && (L->flags & F_MIXED)) ? MHS_FARM_CT : MHS_QC_CT);
+  if (fill_rule(L, pr)) return -1;
+  return 0;
+}
+
+// ----------------------------------------------------------------- batch --
+// The merged plan of a batch of launches (mhs_assemble_batch): every plan
+// item's parameter block, and per (order, variant) its queue segments in
+// launch order.
+struct BatchPlan {
+  std::vector<Params> params;          // one per plan item
+  std::vector<Params> base;            // one per launch (its fold)
+  std::vector<QueueSeg> seg[3][V_COUNT];
+  int64_t total[3][V_COUNT] = {};
+  std::vector<PlanItem> direct[3];     // the Q-family items: ordinary launches
+  size_t off_seg[3][V_COUNT] = {};
+  size_t off_cnt = 0, bytes = 0;
+};
+
+static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+static int batch_plan(const mhs_launch* Ls, int n, BatchPlan& bp) {
+  if (n < 0 || (n > 0 && !Ls)) return fail("mhs_assemble_batch: bad launch array");
+  for (int i = 0; i < n; ++i) {
+    const mhs_launch* L = Ls + i;
+    Params pr;
+    if (make_params(L, pr)) return -1;
+    bp.base.push_back(pr);
+    std::vector<PlanItem> plan;
+    int rc = L->order == 0 ? plan_passes<0>(L, pr, plan) : L->order == 1 ? plan_passes<1>(L, pr, plan)
+                                                                          : plan_passes<2>(L, pr, plan);
+    if (rc) return -1;
+    for (const PlanItem& it : plan) {
+      const int64_t tiles = it.gx * it.gy;
+      if (tiles <= 0) continue;
+      if (it.var == V_QM_S || it.var == V_QM_D) {
+        bp.direct[L->order].push_back(it);
+        continue;
+      }
+      if (it.gx > 0x7fffffffLL || it.gy > 0x7fffffffLL) return fail("mhs_assemble_batch: grid too large");
+      std::vector<QueueSeg>& sg = bp.seg[L->order][it.var];
+      sg.push_back({bp.total[L->order][it.var], (int32_t)bp.params.size(), (int32_t)it.gx});
+      bp.total[L->order][it.var] += tiles;
+      bp.params.push_back(it.pr);
     }
   }
-  if (pr.kind == K_EIGEN) return 0;
-  if ((stages & MHS_STAGE_FOLD) && (pr.flags & F_FOLD) && L->n_strad > 0) {
-    if (!L->fold_list || !L->fold_count || !L->fold_err) return fail("mhs_assemble: fold buffers missing");
-    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, L->fold_count, 2);
-    const int64_t nt = n_rows * L->n_strad;
-    hipLaunchKernelGGL(k_fold_list, dim3(grid_1d(nt)), dim3(256), 0, st, pr, L->strad,
-                       L->n_strad, n_rows, L->fold_list, L->fold_count, L->fold_cap);
-    if (check(hipGetLastError(), "k_fold_list launch")) return -1;
-    if (!L->fold_work || L->fold_work_items < 1) return fail("mhs_assemble: fold work buffer missing");
-    FoldItem* work = (FoldItem*)L->fold_work;
-    const int64_t prep_blocks = (L->fold_work_items + 63) / 64;
-    hipLaunchKernelGGL((k_fold_prep<P>), dim3((unsigned)(prep_blocks < 4096 ? prep_blocks : 4096)), dim3(64), 0, st,
-                       pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items, L->fold_err);
-    if (check(hipGetLastError(), "k_fold_prep launch")) return -1;
-    const dim3 fg(2048);
-    if (stress) {
-      hipLaunchKernelGGL((k_fold<P, true, true>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items);
-      if (pr.mixed_rule)
-        hipLaunchKernelGGL((k_fold<P, true, false, true>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items);
-      else
-        hipLaunchKernelGGL((k_fold<P, true, false>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items);
-    } else {
-      hipLaunchKernelGGL((k_fold<P, false, true>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items);
-      if (pr.mixed_rule)
-        hipLaunchKernelGGL((k_fold<P, false, false, true>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items);
-      else
-        hipLaunchKernelGGL((k_fold<P, false, false>), fg, block, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items);
+  if (bp.params.size() > 0x7fffffff) return fail("mhs_assemble_batch: too many plan items");
+  size_t off = align256(bp.params.size() * sizeof(Params));
+  for (int P = 0; P < 3; ++P)
+    for (int v = 0; v < V_COUNT; ++v) {
+      bp.off_seg[P][v] = off;
+      off = align256(off + bp.seg[P][v].size() * sizeof(QueueSeg));
     }
-    if (check(hipGetLastError(), "k_fold launch")) return -1;
+  bp.off_cnt = off;
+  bp.bytes = off + 3 * V_COUNT * sizeof(unsigned long long);
+  return 0;
+}
+
+// Resident workgroups of each queue kernel per CU, and the CU count, per device.
+static std::mutex g_occ_mu;
+static int g_occ[MHS_MAX_DEVICES][3][V_COUNT];
+static int g_cus[MHS_MAX_DEVICES];
+
+static int queue_grid(int P, int v, int64_t total, unsigned* grid) {
+  int dev = 0;
+  if (check(hipGetDevice(&dev), "mhs_assemble_batch: hipGetDevice")) return -1;
+  if (dev < 0 || dev >= MHS_MAX_DEVICES) return fail("mhs_assemble_batch: device index out of range");
+  std::lock_guard<std::mutex> lock(g_occ_mu);
+  if (!g_cus[dev]) {
+    int cus = 0;
+    if (check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute"))
+      return -1;
+    g_cus[dev] = cus > 0 ? cus : 1;
   }
+  if (!g_occ[dev][P][v]) {
+    int nb = 0;
+    const void* fn = P == 0 ? queue_kernel<0>(v) : P == 1 ? queue_kernel<1>(v) : queue_kernel<2>(v);
+    if (check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, 0), "hipOccupancyMaxActiveBlocksPerMultiprocessor"))
+      return -1;
+    g_occ[dev][P][v] = nb > 0 ? nb : 1;
+  }
+  const int64_t resident = (int64_t)g_occ[dev][P][v] * g_cus[dev];
+  *grid = (unsigned)(total < resident ? total : resident);
   return 0;
 }
 
@@ -1597,7 +1926,6 @@ static_assert(kStaticBytes <= (80u << 20), "static scratch stays modest");
 static_assert(alignof(FoldItem) <= 256, "fold items align inside the scratch");
 static __device__ __align__(256) unsigned char scratch[kStaticBytes];
 
-#define MHS_MAX_DEVICES 64
 static std::mutex g_scratch_mu;
 static unsigned char* g_scratch_ptr[MHS_MAX_DEVICES];
 
@@ -1642,55 +1970,72 @@ int mhs_fill_material(double mu, double lam, double* host_coef) {
 }
 
 int mhs_assemble(const mhs_launch* L, void* stream) {
-  if (!L) return fail("mhs_assemble: null launch descriptor");
-  if (L->abi_version != MHS_ABI_VERSION) return fail("mhs_assemble: ABI version mismatch");
-  if (L->order < 0 || L->order > 2) return fail("mhs_assemble: order must be 0, 1 or 2");
-  if (L->kind < K_DISP || L->kind > K_INTERACTION) return fail("mhs_assemble: unknown kind");
-  if (!L->obs || !L->src || !L->coef || !L->gl_x || !L->gl_w || !L->gq || !L->out)
-    return fail("mhs_assemble: null device pointer");
-  if ((L->kind == K_TRACTION || L->kind == K_INTERACTION) && !L->nrm)
-    return fail("mhs_assemble: traction/interaction need receiver normals");
-  if (L->kind == K_INTERACTION && (!L->rb || L->n_a < 1 || L->n_a > 3 || L->n_b < 1 || L->n_b > 3))
-    return fail("mhs_assemble: interaction needs 1..3 receiver and source components");
-  if (L->kind == K_INTERACTION && !L->sb && L->n_b != 3)
-    return fail("mhs_assemble: identity source basis needs n_b = 3");
-  if ((L->flags & F_PARTS) && L->kind != K_STRESS)
-    return fail("mhs_assemble: parts needs the stress kind");
-  if (L->kind == K_STRESS && (L->flags & (F_PARTS | F_SUBTRACT)) && !L->out2)
-    return fail("mhs_assemble: the stress kind with subtract or parts needs out2 for the eigenstress");
-  if (L->obs_lo < 0 || L->obs_hi > L->n_obs || L->obs_lo > L->obs_hi)
-    return fail("mhs_assemble: bad observer row range");
-  const int K = (L->order + 1) * (L->order + 2) / 2;
-  if (L->n_cols != L->n_src * K) return fail("mhs_assemble: n_cols must be n_src * K");
   Params pr;
-  pr.obs = L->obs; pr.nrm = L->nrm; pr.rb = L->rb; pr.sb = L->sb; pr.src = L->src;
-  pr.coef = L->coef; pr.gl_x = L->gl_x; pr.gl_w = L->gl_w; pr.gq = L->gq; pr.out = L->out; pr.out2 = L->out2;
-  pr.n_obs = L->n_obs; pr.n_src = L->n_src; pr.n_cols = L->n_cols; pr.obs_lo = L->obs_lo;
-  pr.out_lo = L->obs_lo;
-  pr.near_tiles = L->near_tiles;
-  pr.kind = L->kind; pr.flags = L->flags; pr.n_a = L->n_a; pr.n_b = L->n_b;
-  pr.mu = L->mu; pr.lam = L->lam; pr.nu = 0.5 * L->lam / (L->lam + L->mu);
-  {
-    const double c34 = 3.0 - 4.0 * pr.nu;
-    pr.C1 = 1.0 / (16.0 * M_PI * pr.mu * (1.0 - pr.nu));
-    pr.k5e = -6.0 * (1.0 - pr.nu);
-    pr.k7c = 30.0 * (1.0 - pr.nu);
-    pr.k7e = 15.0 / (8.0 * M_PI * pr.C1);
-    pr.k3 = 1.0 - c34;
-    pr.k5c = 3.0 * c34 - 9.0;
-  }
-  if (L->far_tiles < 0 || L->far_tiles > 4096) return fail("mhs_assemble: far_tiles out of range");
-  const bool contracted = use_contracted(L->kind, L->sb != nullptr, L->n_a, L->n_b, L->order, L->flags);
-  pr.na_tot = L->n_a;
-  pr.nb_tot = L->n_b;
-  pr.far_tiles = L->far_tiles ? L->far_tiles : ((contracted && (L->flags & F_MIXED)) ? MHS_FARM_CT : MHS_QC_CT);
-  if (fill_rule(L, pr)) return -1;
+  if (make_params(L, pr)) return -1;
   hipStream_t st = (hipStream_t)stream;
   switch (L->order) {
     case 0: return launch_all<0>(L, pr, st);
     case 1: return launch_all<1>(L, pr, st);
     default: return launch_all<2>(L, pr, st);
   }
+}
+
+int64_t mhs_batch_workspace_bytes(const mhs_launch* launches, int n) {
+  BatchPlan bp;
+  if (batch_plan(launches, n, bp)) return -1;
+  return (int64_t)bp.bytes;
+}
+
+int mhs_assemble_batch(const mhs_launch* launches, int n, void* dev_ws, void* host_ws, int64_t ws_bytes,
+                       void* stream) {
+  BatchPlan bp;
+  if (batch_plan(launches, n, bp)) return -1;
+  if (n == 0) return 0;
+  if (!dev_ws || !host_ws) return fail("mhs_assemble_batch: null workspace");
+  if (ws_bytes < (int64_t)bp.bytes) return fail("mhs_assemble_batch: workspace smaller than mhs_batch_workspace_bytes");
+  if (((uintptr_t)dev_ws | (uintptr_t)host_ws) & 255) return fail("mhs_assemble_batch: workspaces must be 256-byte aligned");
+  hipStream_t st = (hipStream_t)stream;
+  unsigned char* h = (unsigned char*)host_ws;
+  unsigned char* d = (unsigned char*)dev_ws;
+  memcpy(h, bp.params.data(), bp.params.size() * sizeof(Params));
+  for (int P = 0; P < 3; ++P)
+    for (int v = 0; v < V_COUNT; ++v)
+      if (!bp.seg[P][v].empty())
+        memcpy(h + bp.off_seg[P][v], bp.seg[P][v].data(), bp.seg[P][v].size() * sizeof(QueueSeg));
+  memset(h + bp.off_cnt, 0, 3 * V_COUNT * sizeof(unsigned long long));
+  // the tables and zeroed counters in one copy (no memset node under capture)
+  if (check(hipMemcpyAsync(d, h, bp.bytes, hipMemcpyHostToDevice, st), "mhs_assemble_batch: table upload")) return -1;
+  // store passes of every launch, then add passes: one persistent launch per
+  // (order, variant) present
+  for (int stage = 0; stage < 2; ++stage)
+    for (int P = 0; P < 3; ++P)
+      for (int v = 0; v < V_COUNT; ++v) {
+        if (is_store(v) != (stage == 0) || bp.total[P][v] == 0) continue;
+        QueueArgs q;
+        q.params = (const Params*)d;
+        q.seg = (const QueueSeg*)(d + bp.off_seg[P][v]);
+        q.counter = (unsigned long long*)(d + bp.off_cnt) + (P * V_COUNT + v);
+        q.total = bp.total[P][v];
+        q.n_seg = (int32_t)bp.seg[P][v].size();
+        unsigned grid = 0;
+        if (queue_grid(P, v, q.total, &grid)) return -1;
+        const int rc = P == 0 ? launch_queue<0>(v, grid, q, st) : P == 1 ? launch_queue<1>(v, grid, q, st)
+                                                                         : launch_queue<2>(v, grid, q, st);
+        if (rc) return -1;
+      }
+  for (int P = 0; P < 3; ++P)
+    for (const PlanItem& it : bp.direct[P]) {
+      const int rc = P == 0 ? launch_item<0>(it, st) : P == 1 ? launch_item<1>(it, st) : launch_item<2>(it, st);
+      if (rc) return -1;
+    }
+  // each launch's fold (a device-built pair list of its own)
+  for (int i = 0; i < n; ++i) {
+    const mhs_launch* L = launches + i;
+    const int rc = L->order == 0 ? launch_fold<0>(L, bp.base[i], st) : L->order == 1 ? launch_fold<1>(L, bp.base[i], st)
+                                                                                      : launch_fold<2>(L, bp.base[i], st);
+    if (rc) return -1;
+  }
+  return 0;
 }
 
 int mhs_assemble_static(const mhs_launch* L, void* stream) {

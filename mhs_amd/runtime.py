@@ -585,15 +585,18 @@ class Prepared:
         self.far_tiles = best
         return best
 
-    def launch_rows(self, lo: int, hi: int, stages: int = 0):
-        """Enqueue every pass for observer rows [lo, hi) (at most ``self.rows``
-        of them in the continuous layout); no host synchronisation (except
-        the autotuner's first visit of a new launch shape)."""
-        job, lib = self.job, self.dev.lib
+    def _ready(self):
+        """Tune (or fix) the launch shape and the mixed rule before the first
+        launch (the autotuner synchronises on its first visit of a shape)."""
         if self.contracted and not self.far_tiles and not self.tiles_fixed:
             self.tune()
-        if job.mixed and not self.rule_tuned:
+        if self.job.mixed and not self.rule_tuned:
             self.tune_rule()
+
+    def descriptor(self, lo: int, hi: int, stages: int = 0):
+        """The launch descriptor (``_lib.Launch``) of observer rows [lo, hi)
+        (at most ``self.rows`` of them in the continuous layout)."""
+        job = self.job
         if self.cont:
             if hi - lo > self.rows:
                 raise ValueError(f"rows [{lo}, {hi}) exceed the element scratch ({self.rows} rows)")
@@ -603,8 +606,13 @@ class Prepared:
             out_p = C.c_void_p(self.final.data_ptr() + 8 * lo * row_elems)
             out2_p = (C.c_void_p(self.final2.data_ptr() + 8 * lo * row_elems)
                       if job.split else None)
-        L = self._launch_struct(lo, hi, out_p, out2_p, stages)
-        _lib.check(lib.mhs_assemble(C.byref(L), self.handle), lib)
+        return self._launch_struct(lo, hi, out_p, out2_p, stages)
+
+    def epilogue(self, lo: int, hi: int, stages: int = 0):
+        """What follows the assembly of rows [lo, hi) on the stream: the
+        continuous layout's gather, the elastic stress as total minus
+        eigenstress, the strain."""
+        job, lib = self.job, self.dev.lib
         if self.cont:
             self._gather(lo, hi)
         if job.split and job.subtract and (stages == 0 or stages == _ALL_STAGES):
@@ -618,6 +626,16 @@ class Prepared:
             row_elems = int(np.prod(self.final.shape[1:]))
             _lib.check(lib.mhs_strain(C.c_void_p(self.final.data_ptr() + 8 * lo * row_elems), hi - lo,
                                       job.n_dof, job.mu, job.lam, self.handle), lib)
+
+    def launch_rows(self, lo: int, hi: int, stages: int = 0):
+        """Enqueue every pass for observer rows [lo, hi) (at most ``self.rows``
+        of them in the continuous layout); no host synchronisation (except
+        the autotuner's first visit of a new launch shape)."""
+        lib = self.dev.lib
+        self._ready()
+        L = self.descriptor(lo, hi, stages)
+        _lib.check(lib.mhs_assemble(C.byref(L), self.handle), lib)
+        self.epilogue(lo, hi, stages)
 
     def launch(self, stages: int = 0):
         """Enqueue every pass on the stream (no host synchronisation)."""
@@ -682,6 +700,58 @@ class Prepared:
             err = int(self.fold_err.cpu()[0])
             if err:
                 raise RuntimeError(f"mhs_amd: fold grid/list/work overflow (flags {err})")
+
+
+class Batch:
+    """Row ranges of several prepared jobs enqueued as one batch on one
+    stream (``mhs_assemble_batch``): each tiled pass of every range goes
+    into a device work queue drained by one persistent launch per kernel
+    instance, then each range's fold and epilogue follow.  Results are
+    bitwise those of ``Prepared.launch_rows`` on each range.  A job may
+    appear once per batch (its fold buffers and continuous-layout scratch
+    serve one row range at a time); batches on one stream run in order.
+    The workspaces (pinned host, device) are held until :meth:`done`."""
+
+    def __init__(self, items, stream, dev: Device | None = None):
+        dev = dev or device()
+        self.dev, self.stream, self.items = dev, stream, list(items)
+        torch, lib = dev.torch, dev.lib
+        ids = [id(p) for p, _, _ in self.items]
+        if len(set(ids)) != len(ids):
+            raise ValueError("a job may appear once per batch")
+        for prep, _, _ in self.items:
+            prep._ready()
+        n = len(self.items)
+        arr = (_lib.Launch * max(n, 1))()
+        for i, (prep, lo, hi) in enumerate(self.items):
+            arr[i] = prep.descriptor(lo, hi)
+        self._arr = arr
+        nbytes = int(lib.mhs_batch_workspace_bytes(arr, n))
+        if nbytes < 0:
+            _lib.check(-1, lib)
+        nbytes = max(nbytes, 256)
+        # torch's pinned and device blocks are at least 512-byte aligned
+        self.host_ws = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
+        handle = C.c_void_p(stream.cuda_stream)
+        with torch.cuda.stream(stream):
+            self.dev_ws = torch.empty(nbytes, dtype=torch.uint8, device=dev.dev)
+            _lib.check(lib.mhs_assemble_batch(arr, n, C.c_void_p(self.dev_ws.data_ptr()),
+                                              C.c_void_p(self.host_ws.data_ptr()), nbytes, handle), lib)
+            for prep, lo, hi in self.items:
+                if prep.stream != stream:
+                    raise ValueError("every job of a batch must be prepared on the batch's stream")
+                prep.epilogue(lo, hi)
+            self.event = torch.cuda.Event()
+            self.event.record(stream)
+        self.nbytes = nbytes
+
+    def query(self) -> bool:
+        return self.event.query()
+
+    def done(self) -> None:
+        """Wait for the batch and release its workspaces."""
+        self.event.synchronize()
+        self.host_ws = self.dev_ws = None
 
 
 def fold_pairs_bound(obs: np.ndarray, recs: np.ndarray, strad: np.ndarray, chunk: int = 4096) -> int:

@@ -15,7 +15,9 @@ precision.  The loop:
   of bounded pair count and the loop issues one chunk per active request in
   turn (at most ``depth`` in flight each), so a request that arrives while a
   large one is running starts on the device at once and its kernels overlap
-  the large one's on another stream;
+  the large one's on another stream -- or, in the queue mode, the next chunk
+  of every active request goes into one batch whose passes are each drained
+  from a device work queue by one persistent launch (``mhs_assemble_batch``);
 * **keeps results device-resident** (or drops them, for throughput runs) and
   frees the slot and the bytes as soon as a request's last chunk completes,
   admitting the next waiting request.
@@ -208,8 +210,9 @@ class GpuBackend:
     synchronises its stream), as a warm-up does."""
 
     def __init__(self, slots: int, dev: runtime.Device | None = None, headroom: int = SCRATCH_HEADROOM_BYTES,
-                 tune: bool = False):
+                 tune: bool = False, queue: bool = False):
         self.dev = dev or runtime.device()
+        self.queue = bool(queue)
         torch = self.torch = self.dev.torch
         self.total = int(torch.cuda.get_device_properties(self.dev.dev).total_memory)
         self.headroom = int(headroom)
@@ -222,7 +225,8 @@ class GpuBackend:
         torch._C._accelerator_setAllocatorSettings(f"max_split_size_mb:{MAX_SPLIT_MB}")
         torch.cuda.set_per_process_memory_fraction((self.total - self.headroom) / self.total, self.dev.dev)
         self.trim_above = TRIM_ABOVE_BYTES
-        self.streams = [torch.cuda.Stream(device=self.dev.dev) for _ in range(slots)]
+        # the queue mode runs every request on one stream, in batches
+        self.streams = [torch.cuda.Stream(device=self.dev.dev) for _ in range(1 if self.queue else slots)]
         self.tune = bool(tune)
         self.trims = 0          # times the cache was returned to the device
 
@@ -253,7 +257,8 @@ class GpuBackend:
 
     def prepare(self, job: runtime.Job, slot: int, token=None):
         alloc = device_arena(self.dev, self.default_budget()).bump(*token) if token is not None else None
-        prep = runtime.Prepared(job, stream=self.streams[slot], dev=self.dev, alloc=alloc)
+        stream = self.streams[0] if self.queue else self.streams[slot]
+        prep = runtime.Prepared(job, stream=stream, dev=self.dev, alloc=alloc)
         if not self.tune:
             prep.fix_rule()
             prep.fix_tiles()
@@ -267,6 +272,13 @@ class GpuBackend:
             ev = torch.cuda.Event()
             ev.record(prep.stream)
         return ev
+
+    def issue_batch(self, items):
+        """Enqueue one batch of (prep, lo, hi) row ranges (distinct jobs) on
+        the queue stream: one persistent launch per kernel instance drains
+        every range's tiles (runtime.Batch); returns it (query() is True
+        once done)."""
+        return runtime.Batch(items, self.streams[0], self.dev)
 
     def finish(self, prep):
         """(out, out2) of a completed request; raises on a fold overflow."""
@@ -282,13 +294,25 @@ class Server:
     passed by later ones for at most ``bypass_s`` seconds after its arrival,
     and after that by requests smaller than ``small_bytes`` only (default
     1/64 of the budget; bypass_s = small_bytes = 0: strict first come,
-    first served)."""
+    first served).
+
+    ``queue=True``: the admitted requests share one stream and run in
+    batches: each batch takes the next chunk of every active request and
+    enqueues them through ``mhs_assemble_batch``, where each tiled pass of
+    every chunk goes into a device work queue of (request, observer-row
+    tile, column tile) items drained by ONE persistent launch per kernel
+    instance; ``depth`` batches are queued on the stream at a time.
+    ``slots`` then bounds the requests in a batch."""
 
     def __init__(self, slots: int = 4, budget_bytes: int | None = None, chunk_pairs: int = 1 << 25,
-                 depth: int = 2, backend=None, bypass_s: float = 2.0, small_bytes: int | None = None):
+                 depth: int = 2, backend=None, bypass_s: float = 2.0, small_bytes: int | None = None,
+                 queue: bool = False):
         self.slots = int(slots)
+        self.queue = bool(queue)
         self.bypass_s = float(bypass_s)
-        self.backend = backend if backend is not None else GpuBackend(self.slots)
+        self.backend = backend if backend is not None else GpuBackend(self.slots, queue=self.queue)
+        if self.queue and not hasattr(self.backend, "issue_batch"):
+            raise ValueError("queue=True needs a backend with issue_batch")
         self.budget = int(budget_bytes if budget_bytes is not None else self.backend.default_budget())
         self.small_bytes = int(self.budget // 64 if small_bytes is None else small_bytes)
         self.chunk_pairs = int(chunk_pairs)
@@ -325,6 +349,8 @@ class Server:
         # while it plans or prepares, nothing else is admitted or issued)
         st = self.stats = dict.fromkeys(("plan", "prepare", "issue", "finish", "trim", "sleep"), 0.0)
         clock = time.perf_counter
+        rounds = collections.deque()        # queue mode: batches in flight, in stream order
+        self.batches = 0
         t0 = clock()
         while nxt < len(order) or waiting or active:
             now = time.perf_counter() - t0
@@ -381,11 +407,29 @@ class Server:
             # issue: one chunk per active request in turn
             issued = False
             tp = clock()
-            for a in active:
-                while a.pending and len(a.inflight) < self.depth:
-                    lo, hi = a.pending.popleft()
-                    a.inflight.append(be.issue(a.prep, lo, hi))
+            if self.queue:
+                # batches: the next chunk of every active request, all of
+                # them drained by one persistent launch per kernel instance;
+                # up to ``depth`` batches queued on the stream
+                while rounds and rounds[0].query():
+                    rounds.popleft()
+                while len(rounds) < self.depth:
+                    take = [a for a in active if a.pending]
+                    if not take:
+                        break
+                    items = [(a, *a.pending.popleft()) for a in take]
+                    b = be.issue_batch([(a.prep, lo, hi) for a, lo, hi in items])
+                    rounds.append(b)
+                    for a, _, _ in items:
+                        a.inflight.append(b)
                     issued = True
+                    self.batches += 1
+            else:
+                for a in active:
+                    while a.pending and len(a.inflight) < self.depth:
+                        lo, hi = a.pending.popleft()
+                        a.inflight.append(be.issue(a.prep, lo, hi))
+                        issued = True
             st["issue"] += clock() - tp
             # completions
             finished = []

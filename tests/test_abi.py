@@ -20,7 +20,7 @@ def lib():
 
 def test_exports_every_declared_symbol(lib):
     header = (ROOT / "include" / "mhs_amd.h").read_text()
-    declared = set(re.findall(r"^\s*(?:int|const char\*)\s+(mhs_\w+)\(", header, re.M))
+    declared = set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(mhs_\w+)\(", header, re.M))
     assert declared >= {"mhs_assemble", "mhs_gather", "mhs_strain", "mhs_histogram", "mhs_fill_material"}
     for name in declared:
         assert hasattr(lib, name), name
