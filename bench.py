@@ -342,13 +342,15 @@ SERVE_P50_TARGET_S = 0.25
 SERVE_WARM = 48      # requests served one at a time first (tuning the classes they visit)
 
 
-def served_stream(n_req: int, precision: str) -> dict:
+def served_stream(n_req: int, precision: str, queue: bool = False) -> dict:
     """The served stream at its frozen rate: the first SERVE_WARM requests
     served one at a time with tuning on (a warm-up, untimed), then ``n_req``
     requests with Poisson arrivals at SERVE_RATE through the continuous-
     batching loop (4 slots, uncut meshes admitted against the device budget,
     results dropped after completion); p50 / p99 latency, pair-evals/s and
-    the p99 target's pass/fail."""
+    the p99 target's pass/fail.  ``queue``: the loop's queue mode (one
+    stream, batches of every active request's next chunk, each pass drained
+    from a device work queue by one persistent launch)."""
     from dataclasses import replace
     from mhs_amd import serve
     rate = SERVE_RATE[precision]
@@ -356,15 +358,16 @@ def served_stream(n_req: int, precision: str) -> dict:
     _progress(f"served stream {precision}: warm-up on {SERVE_WARM} requests")
     warm = serve.Server(slots=1, backend=serve.GpuBackend(1, tune=True))
     warm.run([replace(r, arrival=0.0) for r in reqs[:SERVE_WARM]], keep=False, realtime=False)
-    _progress(f"served stream {precision}: {n_req} requests at {rate:.2f}/s")
-    be = serve.GpuBackend(4)
-    srv = serve.Server(slots=4, backend=be)
+    sched = "queue" if queue else "streams"
+    _progress(f"served stream {precision} ({sched}): {n_req} requests at {rate:.2f}/s")
+    be = serve.GpuBackend(4, queue=queue)
+    srv = serve.Server(slots=4, backend=be, queue=queue)
     out = srv.run(reqs, keep=False)
     summ = serve.summary(out)
     summ["loop_host_s"] = {k: round(v, 3) for k, v in srv.stats.items()}
     target = SERVE_P99_TARGET_S[precision]
-    summ.update({"precision": precision, "rate_per_s": rate, "rate": "frozen: 0.7 / E[service]",
-                 "slots": 4, "cache_trims": be.trims, "p99_target_s": target,
+    summ.update({"precision": precision, "scheduler": sched, "rate_per_s": rate, "rate": "frozen: 0.7 / E[service]",
+                 "slots": 4, "batches": srv.batches if queue else None, "cache_trims": be.trims, "p99_target_s": target,
                  "p99_pass": bool(summ["p99_s"] <= target and summ["completed"] == n_req),
                  "p50_target_s": SERVE_P50_TARGET_S, "p50_pass": bool(summ["p50_s"] <= SERVE_P50_TARGET_S)})
     return summ
@@ -414,6 +417,18 @@ def _time_mode(mixed, steps, warmup):
     return prep, dt, stage_ms
 
 
+def _served_all(args) -> dict:
+    """The served stream in every requested precision and scheduler, keyed
+    "<precision>" (a stream per request) or "<precision>_queue"."""
+    out = {}
+    for m in (x for x in args.serve_modes.split(",") if x):
+        if args.serve_scheduler in ("streams", "both"):
+            out[m] = served_stream(args.serve_requests, m)
+        if args.serve_scheduler in ("queue", "both"):
+            out[f"{m}_queue"] = served_stream(args.serve_requests, m, queue=True)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -426,6 +441,9 @@ def main() -> int:
                     help="requests of the served stream (0 = skip it)")
     ap.add_argument("--serve-modes", default="mixed,fp64",
                     help="precisions the served stream runs in, comma-separated")
+    ap.add_argument("--serve-scheduler", default="streams", choices=["streams", "queue", "both"],
+                    help="the serving loop's scheduler: a stream per request, the batched work "
+                         "queue, or both on the same stream")
     ap.add_argument("--serve-only", action="store_true",
                     help="run the served stream alone and exit")
     ap.add_argument("--source-major", action="store_true",
@@ -446,8 +464,7 @@ def main() -> int:
               file=sys.stderr)
         return 2
     if args.serve_only:
-        print(json.dumps({"served": {m: served_stream(args.serve_requests, m)
-                                     for m in args.serve_modes.split(",") if m}}), flush=True)
+        print(json.dumps({"served": _served_all(args)}), flush=True)
         return 0
     if args.counter_pass:
         counter_pass(args.mode)
@@ -538,8 +555,7 @@ def main() -> int:
     }
     if args.serve_requests > 0:
         from mhs_amd import serve
-        line["served"] = {m: served_stream(args.serve_requests, m)
-                          for m in args.serve_modes.split(",") if m}
+        line["served"] = _served_all(args)
         serve.release_arenas()
     if not args.no_cpu:
         _progress(f"CPU baseline on {CPU_SHARE} processes")

@@ -73,6 +73,8 @@ class _Active:
         self.index, self.prep, self.slot, self.bytes, self.token = index, prep, slot, nbytes, token
         self.pending = collections.deque(chunks)
         self.inflight = collections.deque()
+        self.cols = prep.job.n_src * prep.job.K                            # element columns
+        self.left = sum((hi - lo) for lo, hi in chunks) * self.cols        # pairs not yet issued
 
 
 #: Device memory the tensor cache may never take while serving: the HIP
@@ -297,18 +299,23 @@ class Server:
     first served).
 
     ``queue=True``: the admitted requests share one stream and run in
-    batches: each batch takes the next chunk of every active request and
-    enqueues them through ``mhs_assemble_batch``, where each tiled pass of
-    every chunk goes into a device work queue of (request, observer-row
-    tile, column tile) items drained by ONE persistent launch per kernel
-    instance; ``depth`` batches are queued on the stream at a time.
-    ``slots`` then bounds the requests in a batch."""
+    batches: each batch takes the next chunk (default 2^23 pairs) of the
+    active requests with the least work left first, up to ``batch_pairs``
+    pairs (at least one chunk), and enqueues them through
+    ``mhs_assemble_batch``, where each tiled pass of every chunk goes into a
+    device work queue of (request, observer-row tile, column tile) items
+    drained by ONE persistent launch per kernel instance; ``depth`` batches
+    are queued on the stream at a time.  ``slots`` then bounds the requests
+    admitted at once."""
 
-    def __init__(self, slots: int = 4, budget_bytes: int | None = None, chunk_pairs: int = 1 << 25,
+    def __init__(self, slots: int = 4, budget_bytes: int | None = None, chunk_pairs: int | None = None,
                  depth: int = 2, backend=None, bypass_s: float = 2.0, small_bytes: int | None = None,
-                 queue: bool = False):
+                 queue: bool = False, batch_pairs: int = 1 << 25):
         self.slots = int(slots)
         self.queue = bool(queue)
+        self.batch_pairs = int(batch_pairs)
+        if chunk_pairs is None:
+            chunk_pairs = (1 << 23) if self.queue else (1 << 25)
         self.bypass_s = float(bypass_s)
         self.backend = backend if backend is not None else GpuBackend(self.slots, queue=self.queue)
         if self.queue and not hasattr(self.backend, "issue_batch"):
@@ -414,10 +421,22 @@ class Server:
                 while rounds and rounds[0].query():
                     rounds.popleft()
                 while len(rounds) < self.depth:
-                    take = [a for a in active if a.pending]
+                    # the requests with the least work left first (the
+                    # small ones finish within a batch or two), then the
+                    # others while the batch has room (at least one chunk)
+                    take = sorted((a for a in active if a.pending), key=lambda a: a.left)
                     if not take:
                         break
-                    items = [(a, *a.pending.popleft()) for a in take]
+                    items, room = [], self.batch_pairs
+                    for a in take:
+                        lo, hi = a.pending[0]
+                        pairs = (hi - lo) * a.cols
+                        if items and pairs > room:
+                            continue
+                        a.pending.popleft()
+                        a.left -= pairs
+                        room -= pairs
+                        items.append((a, lo, hi))
                     b = be.issue_batch([(a.prep, lo, hi) for a, lo, hi in items])
                     rounds.append(b)
                     for a, _, _ in items:

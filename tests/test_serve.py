@@ -283,3 +283,40 @@ def test_server_places_requests_in_the_arena():
     out = serve.Server(slots=4, backend=be, budget_bytes=size).run(reqs, keep=False, realtime=False)
     assert all(s.error is None for s in out)
     assert be.arena.free == [(0, be.arena.size)] and not be.held
+
+
+class FakeQueueBackend(FakeBackend):
+    """The queue mode's batches: each completes 1 ms after issue."""
+
+    def __init__(self):
+        super().__init__()
+        self.batches = []
+
+    def issue_batch(self, items):
+        self.batches.append([(h.serial, lo, hi, h.job.n_src * h.job.K) for h, lo, hi in items])
+        return _Event(time.perf_counter() + 1e-3)
+
+
+def test_queue_mode_batches_least_work_first_within_budget():
+    """Queue mode: every request's rows are issued once, each batch holds at
+    most one chunk per request and stays within batch_pairs (or is a single
+    chunk), and within a batch the requests with the least work left come
+    first."""
+    be = FakeQueueBackend()
+    reqs = [_disp_request(n, 0.0) for n in (200, 24, 120, 40, 8)]
+    srv = serve.Server(slots=5, chunk_pairs=16 * 20, batch_pairs=16 * 50, backend=be, queue=True)
+    out = srv.run(reqs, realtime=False)
+    assert all(s.error is None for s in out)
+    assert srv.batches == len(be.batches) > 1
+    rows = {}
+    for b in be.batches:
+        serials = [h for h, _, _, _ in b]
+        assert len(set(serials)) == len(serials)
+        pairs = sum((hi - lo) * c for _, lo, hi, c in b)
+        assert pairs <= 16 * 50 or len(b) == 1
+        for h, lo, hi, _ in b:
+            assert hi - lo <= 20
+            rows.setdefault(h, []).append((lo, hi))
+    assert sorted(sum(hi - lo for lo, hi in v) for v in rows.values()) == sorted((200, 24, 120, 40, 8))
+    # the first batch starts with the smallest request (8 rows), then 24
+    assert [be.batches[0][i][2] - be.batches[0][i][1] for i in range(2)] == [8, 20]
