@@ -1060,9 +1060,9 @@ kq_far_mixed(QueueArgs q) {
 
 // One pair's near parts as a real call: inlined into the list loop its
 // closed forms spill far more than as a function of its own.
-template <int P>
+template <int P, int NM>
 __device__ __attribute__((noinline)) void near_item_call(const Params& pr, int64_t o, int64_t col) {
-  pair_near_item<P>(pr, o, col);
+  pair_near_item<P, NM>(pr, o, col);
 }
 
 // A block is one receiver row x MHS_ROW_TILE node columns; about one pair in
@@ -1072,7 +1072,7 @@ __device__ __attribute__((noinline)) void near_item_call(const Params& pr, int64
 // columns of a source from one set of closed-form raw moments held in scratch
 // was faster at order 2 while the image edge integrals ran as a memory-bound
 // loop, and slower once they were unrolled: 84 against 66 ms at n = 1560.)
-template <int P>
+template <int P, int NM>
 __device__ __forceinline__ void body_near(const Params& pr, int64_t bx_, int64_t by_) {
   constexpr int K = (P + 1) * (P + 2) / 2;
   __shared__ int list_s[2 * MHS_ROW_TILE];
@@ -1098,17 +1098,17 @@ __device__ __forceinline__ void body_near(const Params& pr, int64_t bx_, int64_t
   for (int i = threadIdx.x; i < n_img + n_dir; i += 256) {
     __asm__ volatile("" ::: "memory");   // no loads hoisted out of the pair loop
     const int c = i < n_img ? list_s[i] : list_s[MHS_ROW_TILE + i - n_img];
-    near_item_call<P>(pr, o, c0 + c);
+    near_item_call<P, NM>(pr, o, c0 + c);
   }
 }
-template <int P>
+template <int P, int NM>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MHS_NEAR_WAVES)))
-k_near(Params pr) { body_near<P>(pr, blockIdx.x, blockIdx.y); }
+k_near(Params pr) { body_near<P, NM>(pr, blockIdx.x, blockIdx.y); }
 // the same pass over the tiles of a batch's queue (see drain_queue)
-template <int P>
+template <int P, int NM>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MHS_NEAR_WAVES)))
 kq_near(QueueArgs q) {
-  drain_queue(q, [&](const Params& pr, int64_t bx, int64_t by) { body_near<P>(pr, bx, by); });
+  drain_queue(q, [&](const Params& pr, int64_t bx, int64_t by) { body_near<P, NM>(pr, bx, by); });
 }
 
 __global__ void k_zero_i32(int* p, int n) {
@@ -1608,7 +1608,8 @@ enum Variant {
   V_FARM_2,      // k_far_mixed, two outputs, store
   V_QM_S,        // k_qm<P, true>: stress family, add
   V_QM_D,        // k_qm<P, false>: displacement, add
-  V_NEAR,        // k_near<P>: contracted near parts, add
+  V_NEAR,        // k_near<P, 2>: contracted near parts, up to 2 x 2 components, add
+  V_NEAR3,       // k_near<P, 3>: the same up to 3 x 3 components, add
   V_COUNT
 };
 static inline bool is_store(int v) { return v <= V_FARM_2; }
@@ -1659,8 +1660,11 @@ static int plan_passes(const mhs_launch* L, const Params& pr, std::vector<PlanIt
             const int var = gout == 1 ? (mixed ? V_FARM_1 : V_FAR64_1) : (mixed ? V_FARM_2 : V_FAR64_2);
             plan.push_back({var, pg, (pr.n_src + per - 1) / per, rows});
           }
-          if (stages & MHS_STAGE_PAIR) adds.push_back({V_NEAR, pg, col_tiles, rows});
         }
+      // the near pass takes every output of the launch at once (its closed-form
+      // tables are formed once per pair and contracted with each output)
+      if (stages & MHS_STAGE_PAIR)
+        adds.push_back({(pr.n_a <= 2 && pr.n_b <= 2) ? V_NEAR : V_NEAR3, pc, col_tiles, rows});
       continue;
     }
     if (stages & MHS_STAGE_PAIR) {
@@ -1689,7 +1693,8 @@ static int launch_item(const PlanItem& it, hipStream_t st) {
     case V_FARM_2: hipLaunchKernelGGL((k_far_mixed<P, 2>), g, b, 0, st, it.pr); break;
     case V_QM_S: hipLaunchKernelGGL((k_qm<P, true>), g, b, 0, st, it.pr); break;
     case V_QM_D: hipLaunchKernelGGL((k_qm<P, false>), g, b, 0, st, it.pr); break;
-    default: hipLaunchKernelGGL((k_near<P>), g, b, 0, st, it.pr); break;
+    case V_NEAR: hipLaunchKernelGGL((k_near<P, 2>), g, b, 0, st, it.pr); break;
+    default: hipLaunchKernelGGL((k_near<P, 3>), g, b, 0, st, it.pr); break;
   }
   return check(hipGetLastError(), "mhs_assemble: pass launch");
 }
@@ -1709,7 +1714,8 @@ static const void* queue_kernel(int v) {
     case V_FARM_2: return (const void*)&kq_far_mixed<P, 2>;
     case V_QM_S:
     case V_QM_D: return nullptr;          // launched per range (see mhs_assemble_batch)
-    default: return (const void*)&kq_near<P>;
+    case V_NEAR: return (const void*)&kq_near<P, 2>;
+    default: return (const void*)&kq_near<P, 3>;
   }
 }
 
@@ -1728,7 +1734,8 @@ static int launch_queue(int v, unsigned grid, const QueueArgs& q, hipStream_t st
     case V_FARM_2: hipLaunchKernelGGL((kq_far_mixed<P, 2>), g, b, 0, st, q); break;
     case V_QM_S:
     case V_QM_D: return fail("mhs_assemble_batch: the Q-family pass has no queue form");
-    default: hipLaunchKernelGGL((kq_near<P>), g, b, 0, st, q); break;
+    case V_NEAR: hipLaunchKernelGGL((kq_near<P, 2>), g, b, 0, st, q); break;
+    default: hipLaunchKernelGGL((kq_near<P, 3>), g, b, 0, st, q); break;
   }
   return check(hipGetLastError(), "mhs_assemble_batch: queue launch");
 }

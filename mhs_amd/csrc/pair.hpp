@@ -361,7 +361,7 @@ MHS_INLINE int near_parts(const Params& pr, const double* __restrict__ src, cons
 // the image side either the folded mirror element (fold band, with its
 // eigenstress) or the closed-form R-family when the mirrored triangle is
 // near.  Everything far is the far pass's.
-template <int P>
+template <int P, int NM>      // NM: the largest receiver / source component count (2 or 3)
 MHS_INLINE void pair_near_item(const Params& pr, int64_t o, int64_t col) {
   constexpr int K = (P + 1) * (P + 2) / 2;
   const int64_t s = col / K;
@@ -375,22 +375,32 @@ MHS_INLINE void pair_near_item(const Params& pr, int64_t o, int64_t col) {
   if (!dnear && !band && !rnear) return;
   const Mat mt{pr.mu, pr.lam, pr.nu};
   const bool sub = (pr.flags & F_SUBTRACT) != 0;
-  const int n_a = pr.n_a, n_b = pr.n_b;     // n_a * n_b <= 2
+  // every output of the launch (up to NM x NM): the weighted tables, the
+  // costly part, are formed once and contracted with each (a, b)
+  const int n_a = pr.n_a, n_b = pr.n_b;
   const double* nr = pr.nrm + 3 * o;
   // receiver halves E_a, source halves F_b (element) and G_b (folded mirror:
   // slip (-sx, -sy, sz) on the mirror normal, fold.py:370-380)
-  double Ea[2][6], Fb[2][6], Gb[2][6], rF[2][2], rG[2][2];
-  for (int a = 0; a < n_a; ++a) half_tensor(pr.rb + ((int64_t)o * pr.na_tot + a) * 3, nr, pr.mu, pr.lam, Ea[a]);
-  for (int b = 0; b < n_b; ++b) {
+  double Ea[NM][6], Fb[NM][6], Gb[NM][6], rF[NM][NM], rG[NM][NM];
+#pragma unroll
+  for (int a = 0; a < NM; ++a)
+    if (a < n_a) half_tensor(pr.rb + ((int64_t)o * pr.na_tot + a) * 3, nr, pr.mu, pr.lam, Ea[a]);
+#pragma unroll
+  for (int b = 0; b < NM; ++b) {
+    if (b >= n_b) continue;
     const double* sb = pr.sb + (s * pr.nb_tot + b) * 3;
     half_tensor(sb, src + S_N, pr.mu, pr.lam, Fb[b]);
     const double sm[3] = {-sb[0], -sb[1], sb[2]};
     half_tensor(sm, src + S_IN, pr.mu, pr.lam, Gb[b]);
-    for (int a = 0; a < n_a; ++a) {
+#pragma unroll
+    for (int a = 0; a < NM; ++a) {
+      if (a >= n_a) continue;
       const double* rb = pr.rb + ((int64_t)o * pr.na_tot + a) * 3;
       double f = 0.0, g = 0.0;
       const int ix[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
+#pragma unroll
       for (int i = 0; i < 3; ++i)
+#pragma unroll
         for (int j = 0; j < 3; ++j) {
           f += rb[i] * Fb[b][ix[i][j]] * nr[j];
           g += rb[i] * Gb[b][ix[i][j]] * nr[j];
@@ -399,16 +409,20 @@ MHS_INLINE void pair_near_item(const Params& pr, int64_t o, int64_t col) {
       rG[a][b] = g;
     }
   }
-  double acc[2] = {0.0, 0.0};
+  double acc[NM * NM];
+#pragma unroll
+  for (int t = 0; t < NM * NM; ++t) acc[t] = 0.0;
   double W[6][21];
   double X1, X2;
   if (dnear) {
     Frame f;
     load_frame(src, false, f);
     const double z = weighted_tables<SpecH<P>, P>(f, src + S_V, k, ob, eps, pr.gl_x, pr.gl_w, W, X1, X2, pr.near_r, rule_of(pr));
-    for (int a = 0; a < n_a; ++a)
-      for (int b = 0; b < n_b; ++b)
-        acc[a * n_b + b] += direct_contract(f, W, z, eps, mt, Ea[a], Fb[b], sub, rF[a][b]);
+#pragma unroll
+    for (int a = 0; a < NM; ++a)
+#pragma unroll
+      for (int b = 0; b < NM; ++b)
+        if (a < n_a && b < n_b) acc[a * NM + b] += direct_contract(f, W, z, eps, mt, Ea[a], Fb[b], sub, rF[a][b]);
   }
   if (band || rnear) {
     Frame g;
@@ -417,9 +431,11 @@ MHS_INLINE void pair_near_item(const Params& pr, int64_t o, int64_t col) {
     reflected_vertices(src, vi);
     if (band) {
       const double z = weighted_tables<SpecH<P>, P>(g, vi, k, ob, eps, pr.gl_x, pr.gl_w, W, X1, X2, pr.near_r, rule_of(pr));
-      for (int a = 0; a < n_a; ++a)
-        for (int b = 0; b < n_b; ++b)
-          acc[a * n_b + b] += direct_contract(g, W, z, eps, mt, Ea[a], Gb[b], sub, rG[a][b]);
+#pragma unroll
+      for (int a = 0; a < NM; ++a)
+#pragma unroll
+        for (int b = 0; b < NM; ++b)
+          if (a < n_a && b < n_b) acc[a * NM + b] += direct_contract(g, W, z, eps, mt, Ea[a], Gb[b], sub, rG[a][b]);
     } else {
       const double zp = weighted_tables<SpecI<P>, P>(g, vi, k, ob, eps, pr.gl_x, pr.gl_w, W, X1, X2, pr.near_r, rule_of(pr));
       const double al[3] = {-g.e1[0], -g.e1[1], -g.e1[2]};
@@ -431,20 +447,27 @@ MHS_INLINE void pair_near_item(const Params& pr, int64_t o, int64_t col) {
       r_ddg(W[2], W[3], W[4], W[5], al, be, ga, ob[2], pr.coef, DDG);
       // rb.H.nr.sb of the R-family = sum over slots E[i][p] F[j][m] DDG[i][j][p][m]
       const int ix[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
-      for (int a = 0; a < n_a; ++a)
-        for (int b = 0; b < n_b; ++b) {
+#pragma unroll
+      for (int a = 0; a < NM; ++a)
+#pragma unroll
+        for (int b = 0; b < NM; ++b) {
+          if (a >= n_a || b >= n_b) continue;
           double v = 0.0;
 #pragma unroll
           for (int t = 0; t < 81; ++t) {
             const int i = t / 27, j = (t / 9) % 3, p = (t / 3) % 3, m = t % 3;
             v = fma(Ea[a][ix[i][p]] * Fb[b][ix[j][m]], DDG[t], v);
           }
-          acc[a * n_b + b] += v;
+          acc[a * NM + b] += v;
         }
     }
   }
   const int64_t oo = o - pr.out_lo;
-  const int os = pr.na_tot * pr.nb_tot;     // entries per (row, column); more in a group launch
-  for (int t = 0; t < n_a * n_b; ++t) pr.out[(oo * pr.n_cols + col) * os + t] += acc[t];
+  const int os = pr.na_tot * pr.nb_tot;     // entries per (row, column)
+#pragma unroll
+  for (int a = 0; a < NM; ++a)
+#pragma unroll
+    for (int b = 0; b < NM; ++b)
+      if (a < n_a && b < n_b) pr.out[(oo * pr.n_cols + col) * os + a * pr.nb_tot + b] += acc[a * NM + b];
 }
 
