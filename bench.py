@@ -488,10 +488,12 @@ def main() -> int:
     prep_o, dt_o, stage_o = _time_mode(not mixed, args.steps, args.warmup)
     ref = prep_o.final.clone()
     hist_o = prep_o.histogram()
+    fold_o = prep_o.fold_histogram()
     del prep_o
     _progress(f"timing {args.mode}")
     prep, dt, stage_ms = _time_mode(mixed, args.steps, args.warmup)
     hist = prep.histogram()
+    fold_h = prep.fold_histogram()
     a, b = (prep.final, ref) if mixed else (ref, prep.final)
     d = (a - b).abs()
     relerr = float((d.max() / b.abs().max()).item())
@@ -520,6 +522,12 @@ def main() -> int:
     roof_fp64 = roofline("k_far (direct + image R + Q far field at the FP64 rule)", 0.0, fl["far"], st_d["far"])
     near_roofs = [roofline("k_near (closed-form near field, mixed rule's switch)", 0.0, flm["near"], st_m["near"]),
                   roofline("k_near (closed-form near field, FP64 rule's switch)", 0.0, fl["near"], st_d["near"])]
+    fh_m, fh_d = (fold_h, fold_o) if mixed else (fold_o, fold_h)
+    ff_m, ff_d = flops.flops_fold(fh_m, True), flops.flops_fold(fh_d, False)
+    fold_roofs = [roofline("k_fold (near and far points; list and grids included), mixed rule",
+                           ff_m["fp32"], ff_m["fp64"], st_m["fold"]),
+                  roofline("k_fold (near and far points; list and grids included), FP64 rule",
+                           0.0, ff_d["fp64"], st_d["fold"])]
     main_roof = roof_mixed if mixed else roof_fp64
     dev = runtime.device()
     line = {
@@ -540,7 +548,7 @@ def main() -> int:
                                "jitter, eps=0.1, Material(30, 30), order 0",
                    "n": n, "pairs": pairs, "mode": args.mode},
         "roofline": dict(main_roof, traffic=_traffic(args.mode)),
-        "roofline_other_kernels": [roof_fp64 if mixed else roof_mixed] + near_roofs,
+        "roofline_other_kernels": [roof_fp64 if mixed else roof_mixed] + near_roofs + fold_roofs,
         "stage_ms": stage_ms,
         "other_mode": {"mode": "fp64" if mixed else "mixed", "ms_per_step": dt_o * 1e3,
                        "value": pairs / dt_o, "stage_ms": stage_o},
@@ -551,7 +559,8 @@ def main() -> int:
                        "tuner": dev.rule_table.get(rule_key, {}),
                        "bound": runtime.RULE_ERROR_BOUND},
         "histogram": hist if mixed else hist_o,
-        "flop": {"fp64_rule": fl, "mixed_rule": flm},
+        "flop": {"fp64_rule": fl, "mixed_rule": flm, "fold_mixed_rule": ff_m, "fold_fp64_rule": ff_d},
+        "fold_histogram": {"mixed_rule": fh_m, "fp64_rule": fh_d},
     }
     if args.serve_requests > 0:
         from mhs_amd import serve

@@ -184,6 +184,15 @@ int mhs_gather(const double* scratch, double* out, int64_t n_lead, int64_t n_ele
  * roofline figures. */
 int mhs_histogram(const mhs_launch* launch, unsigned long long* counts, void* stream);
 
+/* Point counts of the fold's passes (adds into 4 device uint64 counters),
+ * read from the pair list and polar grids that an mhs_assemble of the same
+ * descriptor left in its fold buffers: closed-form (near) points, Gauss
+ * (far) points and their Gauss points (the order squared), each per node
+ * column, and the folded (pair, node) items.  No counterpart in the
+ * reference: the work model of the fold's roofline (fold.py:84-380 is the
+ * algorithm counted). */
+int mhs_fold_histogram(const mhs_launch* launch, unsigned long long* counts, void* stream);
+
 /* a[i] -= b[i] for i < n, device FP64: the elastic stress as the finished
  * total stress minus the finished eigenstress, as matrices.py:287-294 forms
  * it (sig -= star), so stress_matrix == total_stress_matrix -

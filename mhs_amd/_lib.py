@@ -53,6 +53,7 @@ _LIB = None
 EXPORTS = ("mhs_assemble", "mhs_histogram", "mhs_gather", "mhs_strain", "mhs_fill_material", "mhs_ncoef",
            "mhs_src_stride", "mhs_abi_version", "mhs_last_error", "mhs_fold_item_bytes", "mhs_source_major",
            "mhs_subtract", "mhs_assemble_static", "mhs_batch_workspace_bytes", "mhs_assemble_batch",
+           "mhs_fold_histogram",
            "mhs_weld_workspace_bytes", "mhs_shared_nodes", "mhs_tri_geometry")
 
 
@@ -76,6 +77,8 @@ def load(path=None):
     lib.mhs_assemble_batch.argtypes = [C.POINTER(Launch), C.c_int, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
     lib.mhs_assemble_batch.restype = C.c_int
     lib.mhs_histogram.argtypes = [C.POINTER(Launch), C.c_void_p, C.c_void_p]
+    lib.mhs_fold_histogram.argtypes = [C.POINTER(Launch), C.c_void_p, C.c_void_p]
+    lib.mhs_fold_histogram.restype = C.c_int
     lib.mhs_histogram.restype = C.c_int
     lib.mhs_gather.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_int,
                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]

@@ -1355,6 +1355,60 @@ k_fold(Params pr, const int64_t* __restrict__ list,
 }
 
 
+// Point counts of the fold's passes (the work model of their roofline), from
+// the pair list and polar grids a launch left in its fold buffers: per
+// folded (pair, node column) the points the near kernel evaluates in closed
+// form and the far kernel by Gauss tables, and the far points' Gauss points
+// (the rule's order squared), by the kernels' own classification.
+// counts[0] near points, [1] far points, [2] far Gauss points, [3] folded
+// (pair, node) items; added into.
+template <int P>
+__global__ void __launch_bounds__(256) k_fold_hist(Params pr, const int64_t* __restrict__ list,
+                                                   const int* __restrict__ count, int64_t cap,
+                                                   const FoldItem* __restrict__ work, int64_t work_items,
+                                                   unsigned long long* __restrict__ counts) {
+  constexpr int K = (P + 1) * (P + 2) / 2;
+  constexpr int GW = sizeof(FoldGrid) / 8;
+  __shared__ FoldGrid G;
+  __shared__ double u6[6], w6[6], u4[4], w4[4];
+  const int tid = threadIdx.x;
+  int64_t n_items = *count;
+  if (n_items > cap) n_items = cap;
+  if (n_items > work_items) n_items = work_items;
+  if (tid < 6) { u6[tid] = pr.gl_x[15 + tid]; w6[tid] = pr.gl_w[15 + tid]; }
+  if (tid < 4) { u4[tid] = pr.gl_x[6 + tid]; w4[tid] = pr.gl_w[6 + tid]; }
+  unsigned long long nn = 0, nf = 0, ng2 = 0;
+  for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+    const int64_t o = list[2 * item], s = list[2 * item + 1];
+    const double* src = pr.src + s * MHS_SRC_STRIDE;
+    const double xh0 = pr.obs[3 * o], xh1 = pr.obs[3 * o + 1];
+    __syncthreads();
+    {
+      const unsigned long long* gs = reinterpret_cast<const unsigned long long*>(&work[item].G);
+      unsigned long long* gd = reinterpret_cast<unsigned long long*>(&G);
+      for (int w = tid; w < GW; w += blockDim.x) gd[w] = gs[w];
+    }
+    __syncthreads();
+    for (int g = tid; g < G.total; g += 256) {
+      double x, y, w;
+      fold_grid_point(G, xh0, xh1, u6, w6, u4, w4, g, x, y, w);
+      const double op[3] = {x, y, 0.0};
+      const double D = centroid_distance(src + S_C, op, src[S_EPS]);
+      if (D <= pr.near_r * src[S_L]) {
+        ++nn;
+      } else {
+        ++nf;
+        const int ng = pr.mixed_rule ? rule_order(pr.rule, D, src[S_L]) + P : (D > 40.0 * src[S_L] ? 8 : 12);
+        ng2 += (unsigned long long)(ng * ng);
+      }
+    }
+  }
+  if (nn) atomicAdd(counts + 0, nn * K);
+  if (nf) atomicAdd(counts + 1, nf * K);
+  if (ng2) atomicAdd(counts + 2, ng2 * K);
+  if (tid == 0 && blockIdx.x == 0) atomicAdd(counts + 3, (unsigned long long)n_items * K);
+}
+
 // ------------------------------------------------------------- histogram --
 // Branch histogram of the pair set, with the kernels' own classification
 // (the work model behind the roofline figures).  counts[0..12], the FP64
@@ -2108,6 +2162,24 @@ int mhs_source_major(const double* in, double* out, int64_t n_rec, int64_t n_src
   hipLaunchKernelGGL(k_source_major, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, (hipStream_t)stream, in, out,
                      n_rec, n_src, n_a, n_b);
   return check(hipGetLastError(), "k_source_major launch");
+}
+
+int mhs_fold_histogram(const mhs_launch* L, unsigned long long* counts, void* stream) {
+  if (!L || !counts) return fail("mhs_fold_histogram: null pointer");
+  Params pr;
+  if (make_params(L, pr)) return -1;
+  if (!(L->flags & F_FOLD) || L->n_strad <= 0 || L->kind == K_EIGEN) return 0;
+  if (!L->fold_list || !L->fold_count || !L->fold_work || L->fold_work_items < 1)
+    return fail("mhs_fold_histogram: fold buffers missing");
+  const FoldItem* work = (const FoldItem*)L->fold_work;
+  hipStream_t st = (hipStream_t)stream;
+  const dim3 g(2048), b(256);
+  switch (L->order) {
+    case 0: hipLaunchKernelGGL((k_fold_hist<0>), g, b, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items, counts); break;
+    case 1: hipLaunchKernelGGL((k_fold_hist<1>), g, b, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items, counts); break;
+    default: hipLaunchKernelGGL((k_fold_hist<2>), g, b, 0, st, pr, L->fold_list, L->fold_count, L->fold_cap, work, L->fold_work_items, counts); break;
+  }
+  return check(hipGetLastError(), "k_fold_hist launch");
 }
 
 int mhs_histogram(const mhs_launch* L, unsigned long long* counts, void* stream) {

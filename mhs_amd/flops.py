@@ -199,3 +199,40 @@ def flops(h: dict) -> dict:
             + h.get("mirror_far8", 0) * 64 * FLOP_DIRECT_PT + h["fold_pairs"] * FLOP_DIRECT_LIFT)
     return {"pair": float(pair), "q": float(q), "q_fp32_in_mixed": float(q_fp32_mixed),
             "far": float(far), "far_fp32_in_mixed": float(far_fp32), "near": float(near)}
+
+
+# The fold's point kernels (k_fold: the Boussinesq surface integral of the
+# folded pairs; per folded (pair, node column) every polar-grid point),
+# static counts of csrc/fold.hpp and core.hpp at order 0, FMA = 2:
+# * per point, both kinds: the grid point (radius, angle, cos, sin, weight)
+#   ~20, the Boussinesq point stress ~40, the load minus its centre value and
+#   the 18 weighted accumulations ~45                               -> 105
+# * the contraction of one slip direction's surface load (direct_contract:
+#   the E.F products 72, three local quadratics 120, the series over the W
+#   tables 220, the rest ~13)                                      -> 425
+# * near point (closed form): the weighted tables (FLOP_DIRECT_NEAR) + three
+#   contractions + the point                                         -> 3,980
+# * far point, FP64 rule: the frame offsets ~30 + three contractions + the
+#   point, plus per Gauss point the table accumulation FLOP_DIRECT_PT
+# * far point, mixed rule: the point (FP64) plus per Gauss point the direct
+#   series for three slip outputs (FP32: R, 1/R 8, shared 11, 49 per output;
+#   FP64: the point, weight and offsets 26, three accumulations 6 -> 164 FP32 + 32 FP64)
+FLOP_FOLD_POINT = 105
+FLOP_FOLD_CONTRACT = 425
+FLOP_FOLD_NEAR_PT = FLOP_DIRECT_NEAR + 3 * FLOP_FOLD_CONTRACT + FLOP_FOLD_POINT
+FLOP_FOLD_FAR_PT = 30 + 3 * FLOP_FOLD_CONTRACT + FLOP_FOLD_POINT
+FLOP_FOLD_MIX_GP32, FLOP_FOLD_MIX_GP64 = 164, 32
+
+
+def flops_fold(fh: dict, mixed: bool) -> dict:
+    """FLOPs of the fold's point kernels from Prepared.fold_histogram
+    counts (estimated per point from the static counts above)."""
+    near = fh["fold_near_points"] * FLOP_FOLD_NEAR_PT
+    if mixed:
+        fp32 = fh["fold_far_gauss_points"] * FLOP_FOLD_MIX_GP32
+        far64 = fh["fold_far_points"] * FLOP_FOLD_POINT + fh["fold_far_gauss_points"] * FLOP_FOLD_MIX_GP64
+    else:
+        fp32 = 0.0
+        far64 = fh["fold_far_points"] * FLOP_FOLD_FAR_PT + fh["fold_far_gauss_points"] * FLOP_DIRECT_PT
+    return {"near": float(near), "far_fp64": float(far64), "far_fp32": float(fp32),
+            "fp64": float(near + far64), "fp32": float(fp32)}

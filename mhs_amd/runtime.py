@@ -694,6 +694,22 @@ class Prepared:
             vals = cnt.cpu().tolist()
         return dict(zip(self.HIST_KEYS, vals))
 
+    FOLD_HIST_KEYS = ("fold_near_points", "fold_far_points", "fold_far_gauss_points", "fold_items")
+
+    def fold_histogram(self) -> dict:
+        """Point counts of the fold's passes from the last assembly's fold
+        buffers (the rows of its last row chunk; the whole job when it ran
+        as one launch), by the kernels' own classification; syncs."""
+        torch, lib = self.dev.torch, self.dev.lib
+        with torch.cuda.stream(self.stream):
+            cnt = torch.zeros(len(self.FOLD_HIST_KEYS), dtype=torch.int64, device=self.dev.dev)
+            if self.n_strad:
+                lo, hi = self.chunks()[-1]
+                L = self.descriptor(lo, hi)
+                _lib.check(lib.mhs_fold_histogram(C.byref(L), _ptr(cnt), self.handle), lib)
+            vals = cnt.cpu().tolist()
+        return dict(zip(self.FOLD_HIST_KEYS, vals))
+
     def check_fold(self):
         """Raise if a folded pair overflowed its grid or the pair list (syncs)."""
         if self.n_strad:
