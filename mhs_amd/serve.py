@@ -299,9 +299,10 @@ class Server:
     first served).
 
     ``queue=True``: the admitted requests share one stream and run in
-    batches: each batch takes the next chunk (default 2^23 pairs) of the
-    active requests with the least work left first, up to ``batch_pairs``
-    pairs (at least one chunk), and enqueues them through
+    batches: each batch takes the next chunk (default 2^22 pairs) of every
+    active request in admission order (``least_work_first``: those with the
+    least work left first), up to ``batch_pairs`` pairs (default no limit;
+    at least one chunk), and enqueues them through
     ``mhs_assemble_batch``, where each tiled pass of every chunk goes into a
     device work queue of (request, observer-row tile, column tile) items
     drained by ONE persistent launch per kernel instance; ``depth`` batches
@@ -310,12 +311,13 @@ class Server:
 
     def __init__(self, slots: int = 4, budget_bytes: int | None = None, chunk_pairs: int | None = None,
                  depth: int = 2, backend=None, bypass_s: float = 2.0, small_bytes: int | None = None,
-                 queue: bool = False, batch_pairs: int = 1 << 25):
+                 queue: bool = False, batch_pairs: int | None = None, least_work_first: bool = False):
         self.slots = int(slots)
         self.queue = bool(queue)
-        self.batch_pairs = int(batch_pairs)
+        self.least_work_first = bool(least_work_first)
         if chunk_pairs is None:
-            chunk_pairs = (1 << 23) if self.queue else (1 << 25)
+            chunk_pairs = (1 << 22) if self.queue else (1 << 25)
+        self.batch_pairs = int(batch_pairs if batch_pairs is not None else 1 << 62)
         self.bypass_s = float(bypass_s)
         self.backend = backend if backend is not None else GpuBackend(self.slots, queue=self.queue)
         if self.queue and not hasattr(self.backend, "issue_batch"):
@@ -421,10 +423,12 @@ class Server:
                 while rounds and rounds[0].query():
                     rounds.popleft()
                 while len(rounds) < self.depth:
-                    # the requests with the least work left first (the
-                    # small ones finish within a batch or two), then the
-                    # others while the batch has room (at least one chunk)
-                    take = sorted((a for a in active if a.pending), key=lambda a: a.left)
+                    # the active requests in admission order (or with the
+                    # least work left first), one chunk each while the
+                    # batch has room (at least one chunk)
+                    take = [a for a in active if a.pending]
+                    if self.least_work_first:
+                        take.sort(key=lambda a: a.left)
                     if not take:
                         break
                     items, room = [], self.batch_pairs

@@ -304,7 +304,8 @@ def test_queue_mode_batches_least_work_first_within_budget():
     first."""
     be = FakeQueueBackend()
     reqs = [_disp_request(n, 0.0) for n in (200, 24, 120, 40, 8)]
-    srv = serve.Server(slots=5, chunk_pairs=16 * 20, batch_pairs=16 * 50, backend=be, queue=True)
+    srv = serve.Server(slots=5, chunk_pairs=16 * 20, batch_pairs=16 * 50, backend=be, queue=True,
+                       least_work_first=True)
     out = srv.run(reqs, realtime=False)
     assert all(s.error is None for s in out)
     assert srv.batches == len(be.batches) > 1
@@ -320,3 +321,15 @@ def test_queue_mode_batches_least_work_first_within_budget():
     assert sorted(sum(hi - lo for lo, hi in v) for v in rows.values()) == sorted((200, 24, 120, 40, 8))
     # the first batch starts with the smallest request (8 rows), then 24
     assert [be.batches[0][i][2] - be.batches[0][i][1] for i in range(2)] == [8, 20]
+
+
+def test_queue_mode_default_takes_every_request_in_admission_order():
+    """The queue mode's default batch: the next chunk of every active
+    request, in admission order, with no pair budget."""
+    be = FakeQueueBackend()
+    reqs = [_disp_request(n, 0.0) for n in (120, 8, 40)]
+    srv = serve.Server(slots=3, chunk_pairs=16 * 20, backend=be, queue=True)
+    out = srv.run(reqs, realtime=False)
+    assert all(s.error is None for s in out)
+    assert [h for h, _, _, _ in be.batches[0]] == [1, 2, 3]
+    assert [hi - lo for _, lo, hi, _ in be.batches[0]] == [20, 8, 20]
