@@ -161,24 +161,42 @@ def _traffic(mode: str):
 
 
 def counter_pass(mode: str) -> None:
-    """One far pass of the workload (autotuner off, default column tiles),
-    then a 256 MiB fill and a 256 MiB device copy: the program that hardware
-    counter runs (FETCH_SIZE, WRITE_SIZE) profile, the fill and copy being
-    the known-size calibration of those counters."""
+    """The program that hardware counter runs profile: the workload's tuners
+    (column tiles; in the mixed mode also the rule), then one whole step
+    (far, near and fold passes), then a 256 MiB fill and a 256 MiB device
+    copy -- the known-size calibration of FETCH_SIZE / WRITE_SIZE.  The last
+    dispatch of each kernel before the fill is the full-size one.  Prints one
+    JSON line: the modelled FLOPs of that step per kernel (the counted
+    FP32 / FP64 operations are compared against it), the rule and tiles."""
     import torch
-    from mhs_amd import _lib, runtime
-    runtime.autotune = False
-    prep = runtime.Prepared(_job(mode == "mixed"))
+    from mhs_amd import flops, runtime
+    mixed = mode == "mixed"
+    prep = runtime.Prepared(_job(mixed))
     with torch.cuda.stream(prep.stream):
-        prep.launch(stages=_lib.STAGE_Q)
+        prep.launch()                   # the tuners run first
     torch.cuda.synchronize()
+    prep.check_fold()
+    h, fh = prep.histogram(), prep.fold_histogram()
+    ff = flops.flops_fold(fh, mixed)
+    if mixed:
+        fm = flops.flops_mixed(h)
+        model = {"far": {"fp32": fm["far_fp32"], "fp64": fm["far_fp64"]}, "near": {"fp32": 0.0, "fp64": fm["near"]}}
+    else:
+        fl = flops.flops(h)
+        model = {"far": {"fp32": 0.0, "fp64": fl["far"]}, "near": {"fp32": 0.0, "fp64": fl["near"]}}
+    model["fold_near_points"] = {"fp32": 0.0, "fp64": ff["near"]}
+    model["fold_far_points"] = {"fp32": ff["far_fp32"], "fp64": ff["far_fp64"]}
+    dev = runtime.device()
+    key = prep.rule_key()
+    rule = runtime._rule_name(dev.rules[key]) if mixed and key in dev.rules else None
     a = torch.empty(1 << 25, dtype=torch.float64, device="cuda")
     a.fill_(1.0)
     b = torch.empty_like(a)
     torch.cuda.synchronize()
     b.copy_(a)
     torch.cuda.synchronize()
-    _progress(f"counter pass done ({mode}, default column tiles per block)")
+    print(json.dumps({"counter_pass": {"mode": mode, "far_tiles": prep.far_tiles, "rule": rule,
+                                       "model_flop": model, "histogram": h, "fold_histogram": fh}}), flush=True)
 
 
 def source_major_timing() -> list:
@@ -455,8 +473,9 @@ def main() -> int:
     ap.add_argument("--shared-nodes", action="store_true",
                     help="time the shared-node table, host against device, and exit")
     ap.add_argument("--counter-pass", action="store_true",
-                    help="run one far pass of --mode plus a fill and a copy of 256 MiB, print nothing "
-                         "(the program profiled by the FETCH_SIZE / WRITE_SIZE counter runs)")
+                    help="tune, run one whole step of --mode, then a fill and a copy of 256 MiB, and "
+                         "print the step's modelled FLOPs per kernel (the program profiled by the "
+                         "hardware counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the assembly shards by "

@@ -44,9 +44,11 @@ mixed_rule = None
 #: So each row's error is bounded relative to the row, wherever its observer.
 RULE_SAFE = (4.0, ((6.0, 5), (16.0, 4), (100.0, 3), (1e300, 2)))
 
-#: Candidate rules of the tuner for the on-fault interaction, from the fewest
-#: points to the most.  Their low orders far away (one point per direction
-#: beyond 60 L in the first) are far less accurate per pair than RULE_SAFE,
+#: Candidate rules of the tuner for the on-fault interaction: a nearer
+#: near/far switch moves pairs and fold points from the closed forms to
+#: Gauss points, at higher orders close in.  Their low orders far away (one
+#: point per direction beyond 60 L in the first) are far less accurate per
+#: pair than RULE_SAFE,
 #: but the receiver rows of an on-fault interaction each hold their own
 #: element's near field, which the far pairs' errors (decaying with the
 #: stress kernel's 1 / r^3) stay far below: the tuner measures each row's
@@ -54,6 +56,9 @@ RULE_SAFE = (4.0, ((6.0, 5), (16.0, 4), (100.0, 3), (1e300, 2)))
 RULE_CANDIDATES = (
     (4.0, ((8.0, 3), (60.0, 2), (1e300, 1))),
     (4.0, ((10.0, 3), (100.0, 2), (1e300, 1))),
+    (3.0, ((5.0, 4), (10.0, 3), (100.0, 2), (1e300, 1))),
+    (3.0, ((4.0, 5), (8.0, 4), (20.0, 3), (100.0, 2), (1e300, 1))),
+    (2.0, ((3.0, 6), (5.0, 5), (10.0, 4), (30.0, 3), (100.0, 2), (1e300, 1))),
     (5.0, ((12.0, 4), (30.0, 3), (150.0, 2), (1e300, 1))),
     RULE_SAFE,
 )
