@@ -160,7 +160,7 @@ def _traffic(mode: str):
         return json.load(f)["hbm_bytes_per_launch"]
 
 
-def counter_pass(mode: str) -> None:
+def counter_pass(mode: str, workload: str = "c3") -> None:
     """The program that hardware counter runs profile: the workload's tuners
     (column tiles; in the mixed mode also the rule), then one whole step
     (far, near and fold passes), then a 256 MiB fill and a 256 MiB device
@@ -171,7 +171,7 @@ def counter_pass(mode: str) -> None:
     import torch
     from mhs_amd import flops, runtime
     mixed = mode == "mixed"
-    prep = runtime.Prepared(_job(mixed))
+    prep = runtime.Prepared(_job(mixed, workload))
     with torch.cuda.stream(prep.stream):
         prep.launch()                   # the tuners run first
     torch.cuda.synchronize()
@@ -195,7 +195,7 @@ def counter_pass(mode: str) -> None:
     torch.cuda.synchronize()
     b.copy_(a)
     torch.cuda.synchronize()
-    print(json.dumps({"counter_pass": {"mode": mode, "far_tiles": prep.far_tiles, "rule": rule,
+    print(json.dumps({"counter_pass": {"mode": mode, "workload": workload, "far_tiles": prep.far_tiles, "rule": rule,
                                        "model_flop": model, "histogram": h, "fold_histogram": fh}}), flush=True)
 
 
@@ -391,10 +391,32 @@ def served_stream(n_req: int, precision: str, queue: bool = False) -> dict:
     return summ
 
 
-def _job(mixed):
+def _synthetic_tris(kind: str):
+    """Branch-pure meshes for the FLOP-model check of the counter pass.
+    "far": 4,096 triangles of 0.1 km on a 16^3 lattice 2 km apart, 5-35 km
+    deep, each turned at random: every pair but the diagonal is a far pair
+    (20-520 L apart) in both families.  "near": a 12 x 12-cell plane of
+    1 km cells (288 triangles) 50 km deep, dip 40: most direct pairs within
+    the 10 L switch, every image pair far.  Returns (tris, eps)."""
+    from mhs_amd import workloads
+    rng = np.random.default_rng(20261204)
+    if kind == "far":
+        g = np.stack(np.meshgrid(*(np.arange(16.0),) * 3, indexing="ij"), -1).reshape(-1, 3) * 2.0
+        c = g + np.array([-15.0, -15.0, -35.0])
+        shape = np.array([[0.06, 0.0, 0.0], [-0.03, 0.05, 0.0], [-0.03, -0.05, 0.0]])
+        q = rng.normal(size=(c.shape[0], 3, 3))
+        rot = np.linalg.qr(q)[0]
+        tris = c[:, None, :] + np.einsum("nij,kj->nki", rot, shape)
+        return workloads.orient_up(tris), 0.01
+    if kind == "near":
+        return workloads.orient_up(workloads.grid_mesh(workloads.plane(12.0, 12.0, 12, 12, 40.0, -50.0))), 0.1
+    raise ValueError(kind)
+
+
+def _job(mixed, workload: str = "c3"):
     from mhs_amd import runtime, tdcs, workloads
     from mhs_amd.api import collocation_points
-    tris, eps = workloads.c3_interaction()
+    tris, eps = workloads.c3_interaction() if workload == "c3" else _synthetic_tris(workload)
     n = tris.shape[0]
     fr = tdcs.slip_frame(tris)
     return runtime.Job(kind="interaction", order=0, obs=collocation_points(tris, 0), tris=tris,
@@ -466,6 +488,9 @@ def main() -> int:
                     help="run the served stream alone and exit")
     ap.add_argument("--source-major", action="store_true",
                     help="time the device source-major copy of interaction-shaped matrices and exit")
+    ap.add_argument("--workload", default="c3", choices=["c3", "far", "near"],
+                    help="the counter pass's workload: the interaction n=10k, or a branch-pure "
+                         "synthetic mesh (every pair far; most direct pairs near)")
     ap.add_argument("--c4", action="store_true",
                     help="time the three-fault network's continuous-layout order-2 interaction and exit")
     ap.add_argument("--c2", action="store_true",
@@ -486,7 +511,7 @@ def main() -> int:
         print(json.dumps({"served": _served_all(args)}), flush=True)
         return 0
     if args.counter_pass:
-        counter_pass(args.mode)
+        counter_pass(args.mode, args.workload)
         return 0
     if args.shared_nodes:
         print(json.dumps({"shared_nodes": shared_nodes_timing()}))
