@@ -1629,6 +1629,9 @@ __global__ void k_strain(double* __restrict__ buf, int64_t n_obs, int64_t n_cols
 // every form and order (worst measured: the traction of a constant-slip
 // source against 24 x 24 Gauss, 6e-8 at 100 L), so a row's error is bounded
 // relative to the row wherever its observer lies.
+#ifndef MHS_FARM_SINGLE
+#define MHS_FARM_SINGLE 1   // the mixed far pass in one-output groups
+#endif
 #ifndef MHS_FARM_CT
 #define MHS_FARM_CT 2   // default column tiles (of 256 sources) per block of the mixed far pass
 #endif
@@ -1737,17 +1740,22 @@ static int plan_passes(const mhs_launch* L, const Params& pr, std::vector<PlanIt
     if (contracted) {
       // output groups: the whole launch when it has one or two outputs, else
       // one receiver frame row a with source rows b0, b0 + 1 per group (the
-      // pointers offset to the group, the strides those of the whole launch)
-      const bool grouped = nout > 2;
+      // pointers offset to the group, the strides those of the whole launch).
+      // The mixed far pass takes one output per group (gw = 1): its two-output
+      // instance halves the points per chunk and forms each pair's kappas
+      // twice as often, and ran slower than two one-output launches (the
+      // three-fault network at order 2: 20.3 ms against 2 x 8.0 ms a chunk).
+      const int gw = (mixed && MHS_FARM_SINGLE) ? 1 : 2;
+      const bool grouped = nout > gw;
       for (int a = 0; a < (grouped ? pr.n_a : 1); ++a)
-        for (int b0 = 0; b0 < (grouped ? pr.n_b : 1); b0 += 2) {
+        for (int b0 = 0; b0 < (grouped ? pr.n_b : 1); b0 += gw) {
           Params pg = pc;
           if (grouped) {
             pg.rb = pr.rb + a * 3;
             pg.sb = pr.sb + b0 * 3;
             pg.out = pr.out + (a * pr.n_b + b0);
             pg.n_a = 1;
-            pg.n_b = pr.n_b - b0 < 2 ? pr.n_b - b0 : 2;
+            pg.n_b = pr.n_b - b0 < gw ? pr.n_b - b0 : gw;
           }
           const int gout = pg.n_a * pg.n_b;
           // far pass (writes every entry of the group), then the near pass (adds)
