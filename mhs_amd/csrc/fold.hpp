@@ -234,14 +234,130 @@ MHS_INLINE void fold_grid_point(const FoldGrid& G, double xh0, double xh1, const
   w = wr * wt * r;
 }
 
+// The surface load's contraction, split into what a fold item fixes and
+// what a surface point adds.  direct_contract with the receiver half
+// E = half(e_z, e_z) and the source half F_m = half(e_m, n) (slip along axis
+// m, n the source normal) forms the quadratic forms d.A.d of E, F_m and
+// sym(E F_m) in the source frame and its scalar coefficients from those
+// halves alone; only the forms' z terms and the tables W depend on the
+// point.  FoldContract holds the point-independent part for the three slip
+// directions, formed once per item; fold_contract then costs a point about
+// a third of direct_contract, with the same expressions in the same order.
+struct FoldContract {
+  double L[7][6];     // E, F_0..F_2, sym(E F_0)..sym(E F_2): L00, L01, L11, L02, L12, L22 in the frame
+  double A3[3], A5a[3], A5b[3], A5e[3], rFn[3];
+  double A5c, A7c, C1, eigK;
+};
+
+MHS_INLINE void frame_form(const Frame& f, const double A[6], double* L) {
+  const double M[3][3] = {{A[0], A[3], A[4]}, {A[3], A[1], A[5]}, {A[4], A[5], A[2]}};
+  const double* Q[3] = {f.e1, f.e2, f.n};
+  double R[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = i; j < 3; ++j) {
+      double v = 0.0;
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+        v += Q[i][a] * (M[a][0] * Q[j][0] + M[a][1] * Q[j][1] + M[a][2] * Q[j][2]);
+      R[i][j] = v;
+    }
+  L[0] = R[0][0]; L[1] = R[0][1]; L[2] = R[1][1]; L[3] = R[0][2]; L[4] = R[1][2]; L[5] = R[2][2];
+}
+
+// local_quadratic's table from a form in the frame, at plane height z.
+MHS_INLINE void form_at(const double* L, double z, double p[6]) {
+  p[tix(0, 0)] = z * z * L[5];
+  p[tix(1, 0)] = -2.0 * z * L[3];
+  p[tix(0, 1)] = -2.0 * z * L[4];
+  p[tix(2, 0)] = L[0];
+  p[tix(1, 1)] = 2.0 * L[1];
+  p[tix(0, 2)] = L[2];
+}
+
+MHS_INLINE void fold_contract_prep(const Params& pr, const Frame& f, const double* __restrict__ src,
+                                   FoldContract& c) {
+  const double nu = pr.nu;
+  const double c34 = 3.0 - 4.0 * nu;
+  const double eps = src[S_EPS];
+  const double e2 = eps * eps;
+  const double ez[3] = {0.0, 0.0, 1.0};
+  double E[6];
+  half_tensor(ez, ez, pr.mu, pr.lam, E);
+  frame_form(f, E, c.L[0]);
+  const double trE = E[0] + E[1] + E[2];
+  const double Em[3][3] = {{E[0], E[3], E[4]}, {E[3], E[1], E[5]}, {E[4], E[5], E[2]}};
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const double em[3] = {m == 0 ? 1.0 : 0.0, m == 1 ? 1.0 : 0.0, m == 2 ? 1.0 : 0.0};
+    double F[6];
+    half_tensor(em, src + S_N, pr.mu, pr.lam, F);
+    const double Fm[3][3] = {{F[0], F[3], F[4]}, {F[3], F[1], F[5]}, {F[4], F[5], F[2]}};
+    double EF = 0.0, Ms[6];
+    double MM[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        MM[i][j] = Em[i][0] * Fm[0][j] + Em[i][1] * Fm[1][j] + Em[i][2] * Fm[2][j];
+        EF += Em[i][j] * Fm[i][j];
+      }
+    Ms[0] = MM[0][0]; Ms[1] = MM[1][1]; Ms[2] = MM[2][2];
+    Ms[3] = 0.5 * (MM[0][1] + MM[1][0]); Ms[4] = 0.5 * (MM[0][2] + MM[2][0]); Ms[5] = 0.5 * (MM[1][2] + MM[2][1]);
+    const double trF = F[0] + F[1] + F[2];
+    frame_form(f, F, c.L[1 + m]);
+    frame_form(f, Ms, c.L[4 + m]);
+    c.A3[m] = (1.0 - c34) * EF + trE * trF;
+    c.A5a[m] = -3.0 * trF;
+    c.A5b[m] = -3.0 * trE;
+    c.A5e[m] = -6.0 * (1.0 - nu) * e2 * EF;
+    c.rFn[m] = F[2];
+  }
+  c.eigK = 15.0 * e2 * e2 / (8.0 * 3.14159265358979323846);
+  c.A5c = 3.0 * c34 - 9.0;
+  c.A7c = 30.0 * (1.0 - nu) * e2;
+  c.C1 = 1.0 / (16.0 * 3.14159265358979323846 * pr.mu * (1.0 - nu));
+}
+
+// s[m] = 2 direct_contract(f, W, z, eps, mt, E, F_m, true, F_m[2]) for the
+// three slip directions, from the item's FoldContract.
+MHS_INLINE void fold_contract(const FoldContract& c, const double W[6][21], double z, double s[3]) {
+  const double* W3 = W[2];
+  const double* W5 = W[3];
+  const double* W7 = W[4];
+  constexpr int U[6] = {0, 1, 0, 2, 1, 0}, V[6] = {0, 0, 1, 0, 1, 2};   // exponents of tix 0..5
+  double pa[6];
+  form_at(c.L[0], z, pa);
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    double pb[6], pc[6];
+    form_at(c.L[1 + m], z, pb);
+    form_at(c.L[4 + m], z, pc);
+    double S = c.A3[m] * W3[0] + c.A5e[m] * W5[0];
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+      S = fma(c.A5c * pc[t] + c.A5a[m] * pa[t] + c.A5b[m] * pb[t], W5[t], S);
+      S = fma(c.A7c * pc[t], W7[t], S);
+    }
+#pragma unroll
+    for (int t1 = 0; t1 < 6; ++t1)
+#pragma unroll
+      for (int t2 = 0; t2 < 6; ++t2)
+        S = fma(15.0 * pa[t1] * pb[t2], W7[tix(U[t1] + U[t2], V[t1] + V[t2])], S);
+    double out = -c.C1 * S;
+    out -= c.eigK * W7[0] * c.rFn[m];        // the eigenstress term
+    s[m] = 2.0 * out;
+  }
+}
+
 // Surface load s[m] = 2 (H - E C)_zz of node k at (x, y, 0), per slip m
 // (fold.py:214-228).
 template <int P>
-MHS_INLINE void surface_load(const Params& pr, const double* __restrict__ src, int k, double x, double y,
-                             double s[3]) {
+MHS_INLINE void surface_load(const Params& pr, const double* __restrict__ src, const FoldContract& fc, int k,
+                             double x, double y, double s[3]) {
   const double o[3] = {x, y, 0.0};
   const double eps = src[S_EPS];
-  const Mat mt{pr.mu, pr.lam, pr.nu};
   Frame f;
   load_frame(src, false, f);
   double W[6][21];
@@ -249,16 +365,7 @@ MHS_INLINE void surface_load(const Params& pr, const double* __restrict__ src, i
   const double z = weighted_tables<SpecH<P>, P>(f, src + S_V, k, o, eps, pr.gl_x, pr.gl_w, W, X1, X2);
   // sigma_zz (receiver half of e_z on the plane z = 0) per unit slip e_m,
   // minus the eigenstress E C_zz, by the contracted lift
-  const double ez[3] = {0.0, 0.0, 1.0};
-  double E[6];
-  half_tensor(ez, ez, pr.mu, pr.lam, E);
-#pragma unroll
-  for (int m = 0; m < 3; ++m) {
-    const double em[3] = {m == 0 ? 1.0 : 0.0, m == 1 ? 1.0 : 0.0, m == 2 ? 1.0 : 0.0};
-    double Fm[6];
-    half_tensor(em, src + S_N, pr.mu, pr.lam, Fm);
-    s[m] = 2.0 * direct_contract(f, W, z, eps, mt, E, Fm, true, Fm[2]);
-  }
+  fold_contract(fc, W, z, s);
 }
 
 
@@ -267,11 +374,10 @@ MHS_INLINE void surface_load(const Params& pr, const double* __restrict__ src, i
 // kernel evaluating far points alone carries none of the closed form's
 // registers.
 template <int P>
-MHS_INLINE void surface_load_far(const Params& pr, const double* __restrict__ src, int k, double x, double y,
-                                 double s[3]) {
+MHS_INLINE void surface_load_far(const Params& pr, const double* __restrict__ src, const FoldContract& fc, int k,
+                                 double x, double y, double s[3]) {
   const double o[3] = {x, y, 0.0};
   const double eps = src[S_EPS];
-  const Mat mt{pr.mu, pr.lam, pr.nu};
   Frame f;
   load_frame(src, false, f);
   const double d0 = o[0] - f.c[0], d1 = o[1] - f.c[1], d2 = o[2] - f.c[2];
@@ -283,14 +389,5 @@ MHS_INLINE void surface_load_far(const Params& pr, const double* __restrict__ sr
   const int off = ng * (ng - 1) / 2;
   double W[6][21];
   weighted_far<SpecH<P>, P>(f, k, X1, X2, h2, ng, pr.gl_x + off, pr.gl_w + off, W);
-  const double ez[3] = {0.0, 0.0, 1.0};
-  double E[6];
-  half_tensor(ez, ez, pr.mu, pr.lam, E);
-#pragma unroll
-  for (int m = 0; m < 3; ++m) {
-    const double em[3] = {m == 0 ? 1.0 : 0.0, m == 1 ? 1.0 : 0.0, m == 2 ? 1.0 : 0.0};
-    double Fm[6];
-    half_tensor(em, src + S_N, pr.mu, pr.lam, Fm);
-    s[m] = 2.0 * direct_contract(f, W, z, eps, mt, E, Fm, true, Fm[2]);
-  }
+  fold_contract(fc, W, z, s);
 }

@@ -1206,7 +1206,13 @@ __global__ void __launch_bounds__(64) k_fold_prep(Params pr, const int64_t* __re
     fold_grid_build(it.G, xh0, xh1, zeta, src[S_CREASE], src[S_CREASE + 1], src[S_CREASE_R], src[S_EPS],
                     src[S_DIAM]);
     if (it.G.overflow) atomicOr(err, 1);
-    for (int k = 0; k < K; ++k) surface_load<P>(pr, src, k, xh0, xh1, it.s0[k]);
+    FoldContract fc;
+    {
+      Frame f;
+      load_frame(src, false, f);
+      fold_contract_prep(pr, f, src, fc);
+    }
+    for (int k = 0; k < K; ++k) surface_load<P>(pr, src, fc, k, xh0, xh1, it.s0[k]);
   }
 }
 
@@ -1214,8 +1220,9 @@ __global__ void __launch_bounds__(64) k_fold_prep(Params pr, const int64_t* __re
 // point loop it spills more and runs slower.
 template <int P>
 __device__ __attribute__((noinline)) void surface_load_call(const Params& pr, const double* __restrict__ src,
-                                                            int k, double x, double y, double* sl) {
-  surface_load<P>(pr, src, k, x, y, sl);
+                                                            const FoldContract& fc, int k, double x, double y,
+                                                            double* sl) {
+  surface_load<P>(pr, src, fc, k, x, y, sl);
 }
 
 // The Boussinesq sum of the folded pairs over their polar grids, split by
@@ -1244,6 +1251,7 @@ k_fold(Params pr, const int64_t* __restrict__ list,
   static_assert(sizeof(FoldGrid) % 8 == 0, "FoldGrid copies as 8-byte words");
   __shared__ FoldGrid G;
   __shared__ float kdz_s[MIX ? 3 * MHS_DC : 1];   // the item's direct-series coefficients, per slip
+  __shared__ FoldContract fc_s;                    // the item's surface-load contraction (not MIX)
   __shared__ double sh[3];
   __shared__ double red[NA][8];
   __shared__ double u6[6], w6[6], u4[4], w4[4];
@@ -1275,6 +1283,11 @@ k_fold(Params pr, const int64_t* __restrict__ list,
       half_tensor(ez, ez, pr.mu, pr.lam, E);
       half_tensor(em, src + S_N, pr.mu, pr.lam, Fm);
       direct_series_coefs(pr, E, Fm, src[S_EPS], Fm[2], true, kdz_s + tid * MHS_DC);
+    }
+    if (!MIX && tid == 0) {
+      Frame f;
+      load_frame(src, false, f);
+      fold_contract_prep(pr, f, src, fc_s);
     }
     __syncthreads();
     for (int k = 0; k < K; ++k) {
@@ -1321,7 +1334,7 @@ k_fold(Params pr, const int64_t* __restrict__ list,
           fold_grid_point(G, xh0, xh1, u6, w6, u4, w4, g, x, y, w);
           double sl[3];
           if constexpr (NEAR) {
-            surface_load_call<P>(pr, src, k, x, y, sl);
+            surface_load_call<P>(pr, src, fc_s, k, x, y, sl);
           } else if constexpr (MIX) {
             const double op[3] = {x, y, 0.0};
             const int ng = rule_order(pr.rule, centroid_distance(src + S_C, op, src[S_EPS]), src[S_L]) + P;
@@ -1333,7 +1346,7 @@ k_fold(Params pr, const int64_t* __restrict__ list,
             for (int kk = 0; kk < K; ++kk)
               if (kk == k) { sl[0] = am[kk][0]; sl[1] = am[kk][1]; sl[2] = am[kk][2]; }
           } else {
-            surface_load_far<P>(pr, src, k, x, y, sl);
+            surface_load_far<P>(pr, src, fc_s, k, x, y, sl);
           }
           sl[0] -= s0; sl[1] -= s1; sl[2] -= s2;
           const double dx = xh0 - x, dy = xh1 - y;
