@@ -73,8 +73,11 @@
 #ifndef MHS_PAIR_WAVES
 #define MHS_PAIR_WAVES 2       // the pair pass of the displacement form
 #endif
+// The fold's far-point kernel (Gauss tables, or the mixed rule's direct
+// series) at 3 waves per SIMD: at n = 10^4 its mixed-rule pass ran 1.7 ms
+// faster than at 2 and 0.9 ms faster than at 4.
 #ifndef MHS_FOLD_FAR_WAVES
-#define MHS_FOLD_FAR_WAVES 2   // occupancy target of the fold's far-point kernel
+#define MHS_FOLD_FAR_WAVES 3   // occupancy target of the fold's far-point kernel
 #endif
 #ifndef MHS_QC_CT
 #define MHS_QC_CT 16   // default column tiles per block of the far pass
@@ -1665,8 +1668,12 @@ static Rule default_rule() {
 // row with up to two source rows each) are no more than the node columns K
 // -- measured: the three-fault network at order 2 (3 x 2 components, K = 6)
 // 1.03 s -> 0.55 s, the Gaussian-field fault at order 0 (3 x 3, K = 1)
-// 10 ms -> 16 ms.  The FP64 mode keeps the tensor passes beyond two
-// components (faster there: 2.2 s against 2.8 s on the network).
+// 10 ms -> 16 ms.  With the far pass in one-output launches the rule still
+// holds: every mixed 3 x 3 interaction contracted ran the Gaussian-field
+// fault at order 0 in 20.8 ms against 11.4 ms, and a 1,440-triangle fault
+// in 17.7 / 69.2 / 229 ms against 8.3 / 59.7 / 227 ms at orders 0 / 1 / 2.
+// The FP64 mode keeps the tensor passes beyond two components (faster
+// there: 2.2 s against 2.8 s on the network).
 static bool use_contracted(int kind, bool has_sb, int n_a, int n_b, int order, int flags) {
   if (kind != K_INTERACTION || !has_sb) return false;
   if (n_a * n_b <= 2) return true;
