@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define MHS_ABI_VERSION 5
+#define MHS_ABI_VERSION 6
 
 #define MHS_GQ_TOTAL 2377760   /* sum of n^2 for n = 1..192 */
 
@@ -66,12 +66,15 @@ enum { MHS_STAGE_PAIR = 1, MHS_STAGE_Q = 2, MHS_STAGE_FOLD = 4 };
  * interaction, whose rows each hold their own element's near field); the
  * mixed mode's bound is 1e-6 of each row's largest entry.  The all-FP64
  * mode always uses the reference's rule (fullspace/defaults.py:20-23,
- * kernels/image.py:249-254). */
+ * kernels/image.py:249-254).  fold_near_ratio sets the same switch for the
+ * fold's surface points alone (closed form within it, the bands' Gauss
+ * orders beyond); 0 takes near_ratio. */
 #define MHS_RULE_BANDS 8
 typedef struct mhs_rule {
   double near_ratio;
   double band_hi[MHS_RULE_BANDS];   /* ascending */
   int32_t order[MHS_RULE_BANDS];    /* 1 .. 190 */
+  double fold_near_ratio;           /* 0 or in [1, 10] */
 } mhs_rule;
 
 typedef struct mhs_launch {

@@ -10,7 +10,7 @@ import ctypes as C
 import pathlib
 
 LIB_PATH = pathlib.Path(__file__).resolve().parent / "libmhs_amd.so"
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 KIND = {"disp": 0, "stress": 1, "eigen": 2, "traction": 3, "interaction": 4}
 F_FOLD, F_SUBTRACT, F_PARTS, F_MIXED = 1, 2, 4, 8
@@ -25,7 +25,7 @@ class Rule(C.Structure):
     """mirror of ``mhs_rule`` (the mixed mode's far-field rule; near_ratio 0:
     the library's default)"""
     _fields_ = [("near_ratio", C.c_double), ("band_hi", C.c_double * RULE_BANDS),
-                ("order", C.c_int32 * RULE_BANDS)]
+                ("order", C.c_int32 * RULE_BANDS), ("fold_near_ratio", C.c_double)]
 
 
 class Launch(C.Structure):

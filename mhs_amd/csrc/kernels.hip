@@ -1312,7 +1312,7 @@ k_fold(Params pr, const int64_t* __restrict__ list,
             double x, y, w;
             fold_grid_point(G, xh0, xh1, u6, w6, u4, w4, g, x, y, w);
             const double op[3] = {x, y, 0.0};
-            take = (centroid_distance(src + S_C, op, src[S_EPS]) <= pr.near_r * src[S_L]) == NEAR;
+            take = (centroid_distance(src + S_C, op, src[S_EPS]) <= pr.fold_near_r * src[S_L]) == NEAR;
           }
           const unsigned long long m = __ballot(take);
           if ((tid & 63) == 0) wcnt_s[wave] = __popcll(m);
@@ -1454,7 +1454,7 @@ __global__ void __launch_bounds__(256) k_fold_hist(Params pr, const int64_t* __r
       fold_grid_point(G, xh0, xh1, u6, w6, u4, w4, g, x, y, w);
       const double op[3] = {x, y, 0.0};
       const double D = centroid_distance(src + S_C, op, src[S_EPS]);
-      if (D <= pr.near_r * src[S_L]) {
+      if (D <= pr.fold_near_r * src[S_L]) {
         ++nn;
       } else {
         ++nf;
@@ -1688,6 +1688,7 @@ static bool use_contracted(int kind, bool has_sb, int n_a, int n_b, int order, i
 static int fill_rule(const mhs_launch* L, Params& pr) {
   const bool mixed_rule = (L->flags & F_MIXED) != 0;
   pr.near_r = 10.0;
+  pr.fold_near_r = 10.0;
   pr.rule = default_rule();
   pr.mixed_rule = mixed_rule ? 1 : 0;
   if (!mixed_rule) return 0;
@@ -1702,8 +1703,12 @@ static int fill_rule(const mhs_launch* L, Params& pr) {
       pr.rule.n[b] = r.order[b];
     }
     pr.rule.near = r.near_ratio;
+    if (r.fold_near_ratio != 0.0 && !(r.fold_near_ratio >= 1.0 && r.fold_near_ratio <= 10.0))
+      return fail("mhs_assemble: rule.fold_near_ratio must be in [1, 10] (or 0 for near_ratio)");
   }
   pr.near_r = pr.rule.near;
+  pr.fold_near_r = (L->rule.near_ratio != 0.0 && L->rule.fold_near_ratio != 0.0) ? L->rule.fold_near_ratio
+                                                                                 : pr.near_r;
   return 0;
 }
 

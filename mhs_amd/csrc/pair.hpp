@@ -39,6 +39,7 @@ struct Params {
   // near/far switch of the contracted interaction's passes, in units of L:
   // 10 (the reference's D* = 10 L), or the mixed rule's near ratio
   double near_r;
+  double fold_near_r;    // the same switch for the fold's surface points
   Rule rule;             // the mixed mode's far-pass rule
   int mixed_rule;        // 1: the launch runs the mixed rule (mixed mode, contracted interaction)
   // per 256 element columns: bounding box of the sources' centroids and

@@ -29,8 +29,9 @@ def tile_candidates(mixed: bool):
 
 
 #: The mixed mode's far-field rule, as (near_ratio, ((band_hi, order), ...))
-#: in units of the source's size L (Gauss points per direction: order + the
-#: shape order): None lets the autotuner choose among RULE_CANDIDATES for an
+#: or (near_ratio, bands, fold_near_ratio), in units of the source's size L
+#: (Gauss points per direction: order + the shape order): None lets the
+#: autotuner choose among RULE_CANDIDATES for an
 #: on-fault interaction and takes the library's default, RULE_SAFE, for
 #: everything else.  See include/mhs_amd.h (mhs_rule).
 mixed_rule = None
@@ -46,7 +47,8 @@ RULE_SAFE = (4.0, ((6.0, 5), (16.0, 4), (100.0, 3), (1e300, 2)))
 
 #: Candidate rules of the tuner for the on-fault interaction: a nearer
 #: near/far switch moves pairs and fold points from the closed forms to
-#: Gauss points, at higher orders close in.  Their low orders far away (one
+#: Gauss points, at higher orders close in; a third entry sets the fold's
+#: surface points' switch alone (the first band's order beyond it).  Their low orders far away (one
 #: point per direction beyond 60 L in the first) are far less accurate per
 #: pair than RULE_SAFE,
 #: but the receiver rows of an on-fault interaction each hold their own
@@ -57,6 +59,8 @@ RULE_CANDIDATES = (
     (4.0, ((8.0, 3), (60.0, 2), (1e300, 1))),
     (4.0, ((10.0, 3), (100.0, 2), (1e300, 1))),
     (3.0, ((5.0, 4), (10.0, 3), (100.0, 2), (1e300, 1))),
+    (3.0, ((5.0, 4), (10.0, 3), (100.0, 2), (1e300, 1)), 2.0),
+    (3.0, ((5.0, 4), (10.0, 3), (100.0, 2), (1e300, 1)), 1.5),
     (3.0, ((4.0, 5), (8.0, 4), (20.0, 3), (100.0, 2), (1e300, 1))),
     (2.0, ((3.0, 6), (5.0, 5), (10.0, 4), (30.0, 3), (100.0, 2), (1e300, 1))),
     (5.0, ((12.0, 4), (30.0, 3), (150.0, 2), (1e300, 1))),
@@ -69,17 +73,20 @@ RULE_ERROR_BOUND = 1e-7
 
 
 def _rule_name(rule) -> str:
-    near, bands = rule
+    near, bands = rule[0], rule[1]
+    fold = f"; fold points near {rule[2]:g} L" if len(rule) > 2 else ""
     return f"near {near:g} L; " + ", ".join(f"n={n} to {hi:g} L" if hi < 1e299 else f"n={n} beyond"
-                                           for hi, n in bands)
+                                           for hi, n in bands) + fold
 
 
 def rule_struct(rule):
-    """A :class:`_lib.Rule` from (near_ratio, [(band_hi, order), ...])."""
+    """A :class:`_lib.Rule` from (near_ratio, [(band_hi, order), ...]) or
+    (near_ratio, bands, fold_near_ratio)."""
     r = _lib.Rule()
     if rule is None:
         return r
-    near, bands = rule
+    near, bands = rule[0], rule[1]
+    r.fold_near_ratio = float(rule[2]) if len(rule) > 2 else 0.0
     if not 1 <= len(bands) <= _lib.RULE_BANDS:
         raise ValueError(f"a rule has 1..{_lib.RULE_BANDS} bands")
     bands = list(bands) + [(1e300, bands[-1][1])] * (_lib.RULE_BANDS - len(bands))

@@ -31,14 +31,15 @@ def test_sizes(lib):
     assert lib.mhs_src_stride() == geometry.SRC_STRIDE
     assert lib.mhs_ncoef() == 2167
     from mhs_amd import _lib
-    assert lib.mhs_abi_version() == _lib.ABI_VERSION == 5
+    assert lib.mhs_abi_version() == _lib.ABI_VERSION == 6
     # the launch descriptor mirror has the header's layout: 31 fields, the
     # flattened Gauss rule pointer right after the 1-D tables
     names = [f[0] for f in _lib.Launch._fields_]
     assert names.index("gq") == names.index("gl_w") + 1
     assert geometry.GQ_TOTAL == sum(n * n for n in range(1, 193))
     assert names[-4:] == ["fold_work", "fold_work_items", "rule", "near_tiles"]
-    assert C.sizeof(_lib.Rule) == 8 + 8 * _lib.RULE_BANDS + 4 * _lib.RULE_BANDS
+    assert C.sizeof(_lib.Rule) == 8 + 8 * _lib.RULE_BANDS + 4 * _lib.RULE_BANDS + 8
+    assert _lib.Rule.fold_near_ratio.offset == 8 + 12 * _lib.RULE_BANDS
     from mhs_amd import runtime
     for p in range(3):
         assert 0 < lib.mhs_fold_item_bytes(p) <= runtime.FOLD_ITEM_BYTES
@@ -78,3 +79,16 @@ def test_material_coefficients_match_oracle(lib):
         for rec in t[key]:
             want.append(sum((num / den) * nu ** pnu for _, pnu, num, den in rec[-1]) * scale)
     assert np.allclose(c, np.array(want), rtol=1e-14, atol=0)
+
+
+def test_rule_struct_fold_switch():
+    """A rule's optional third entry is the fold points' own switch; two
+    entries leave it 0 (the library then takes near_ratio)."""
+    from mhs_amd import runtime
+    two = runtime.rule_struct((3.0, ((5.0, 4), (1e300, 1))))
+    three = runtime.rule_struct((3.0, ((5.0, 4), (1e300, 1)), 2.0))
+    assert two.near_ratio == three.near_ratio == 3.0
+    assert two.fold_near_ratio == 0.0 and three.fold_near_ratio == 2.0
+    assert list(three.order)[:3] == [4, 1, 1] and three.band_hi[0] == 5.0
+    assert runtime._rule_name((3.0, ((5.0, 4), (1e300, 1)), 2.0)).endswith("fold points near 2 L")
+    assert "fold" not in runtime._rule_name((3.0, ((5.0, 4), (1e300, 1))))
