@@ -61,6 +61,8 @@ RULE_CANDIDATES = (
     (3.0, ((5.0, 4), (10.0, 3), (100.0, 2), (1e300, 1))),
     (3.0, ((5.0, 4), (10.0, 3), (100.0, 2), (1e300, 1)), 2.0),
     (3.0, ((5.0, 4), (10.0, 3), (100.0, 2), (1e300, 1)), 1.5),
+    (2.5, ((5.0, 4), (10.0, 3), (100.0, 2), (1e300, 1)), 1.5),
+    (2.0, ((5.0, 4), (10.0, 3), (100.0, 2), (1e300, 1)), 1.5),
     (3.0, ((4.0, 5), (8.0, 4), (20.0, 3), (100.0, 2), (1e300, 1))),
     (2.0, ((3.0, 6), (5.0, 5), (10.0, 4), (30.0, 3), (100.0, 2), (1e300, 1))),
     (5.0, ((12.0, 4), (30.0, 3), (150.0, 2), (1e300, 1))),
