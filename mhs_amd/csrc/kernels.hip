@@ -40,13 +40,15 @@
 // Lanes per pair of the far pass: fewer lanes put more pairs in a wave, so
 // the per-pair set-up is issued once for more pairs, until the per-block
 // kappa tables in LDS cut the occupancy (measured at n = 10^4: 8 lanes beat
-// 16 by 13 % with FP32 points and by 20 % with FP64 points; 4 lanes lose
-// with FP32 points, LDS-limited to 2 waves per SIMD).
+// 16 by 13 % with FP32 points and by 20 % with FP64 points, both at 2 waves
+// per SIMD; 4 lanes lose with FP32 points, LDS-limited to 2 waves per SIMD).
+// With FP64 points 16 lanes halve the per-block kappa tables (79 -> 40 KB),
+// so 3 waves per SIMD fit: FP64 far pass 183.7 -> 175.4 ms (4 waves: 188).
 #ifndef MHS_FAR_LPP32
 #define MHS_FAR_LPP32 8   // FP32 points, one output
 #endif
 #ifndef MHS_FAR_LPP64
-#define MHS_FAR_LPP64 8   // FP64 points, one output
+#define MHS_FAR_LPP64 16  // FP64 points, one output
 #endif
 #ifndef MHS_FAR_LPP32B
 #define MHS_FAR_LPP32B 16 // FP32 points, two outputs
@@ -58,7 +60,7 @@
 #define MHS_FAR_WAVES32 3 // occupancy target of the far pass, FP32 points (waves per SIMD)
 #endif
 #ifndef MHS_FAR_WAVES64
-#define MHS_FAR_WAVES64 2 // occupancy target of the far pass, FP64 points
+#define MHS_FAR_WAVES64 3 // occupancy target of the far pass, FP64 points
 #endif
 // Occupancy targets of the closed-form kernels, whose per-item work is a
 // called function: 2 waves per SIMD beat 1 (latency of the long FP64 chains)
