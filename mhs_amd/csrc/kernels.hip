@@ -35,7 +35,7 @@
 #define MHS_FAR_NPT32 4   // quadrature points per lane of the far pass, FP32 points
 #endif
 #ifndef MHS_FAR_NPT64
-#define MHS_FAR_NPT64 2   // quadrature points per lane of the far pass, FP64 points
+#define MHS_FAR_NPT64 2   // quadrature points per lane of the far pass, FP64 points (4 spilled: 361 against 172 ms at n = 10^4)
 #endif
 // Lanes per pair of the far pass: fewer lanes put more pairs in a wave, so
 // the per-pair set-up is issued once for more pairs, until the per-block
@@ -800,6 +800,9 @@ kq_far(QueueArgs q) {
 #ifndef MHS_FARM_NPT
 #define MHS_FARM_NPT 4   // points per chunk of the mixed far pass
 #endif
+#ifndef MHS_FARM_NPT_MULTI
+#define MHS_FARM_NPT_MULTI 2   // the same with two or more outputs (the fold's three slips; 1 and 4 ran no faster)
+#endif
 
 // kappa of output o, monomial k: sum_v F[o][v] B[o][8 k + v] (B a receiver
 // table in LDS, rows of 8 floats of which 6 are used).
@@ -925,7 +928,7 @@ template <int P, int NOUT, int FAM, typename KapT>
 __device__ __forceinline__ void mixed_family(const Params& pr, const double* __restrict__ src, const double ob[3],
                                              int ng, const float* kd, const KapT& kap, double wscale,
                                              double (*acc)[NOUT]) {
-  constexpr int NPT = NOUT == 1 ? MHS_FARM_NPT : (MHS_FARM_NPT + 1) / 2;   // two outputs: half the points
+  constexpr int NPT = NOUT == 1 ? MHS_FARM_NPT : MHS_FARM_NPT_MULTI;
   if (ng == 1) mixed_pass<P, NOUT, 1, FAM>(pr, src, ob, 1, kd, kap, wscale, acc);
   else mixed_pass<P, NOUT, NPT, FAM>(pr, src, ob, ng, kd, kap, wscale, acc);
 }
