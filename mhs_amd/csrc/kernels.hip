@@ -65,7 +65,9 @@
 // Occupancy targets of the closed-form kernels, whose per-item work is a
 // called function: 2 waves per SIMD beat 1 (latency of the long FP64 chains)
 // and 3 or 4 (spills), measured at n = 10^4: fold near points 47 -> 37 ms,
-// near pass 18 -> 12.7 ms.
+// near pass 18 -> 12.7 ms; measured again on the current kernels (mixed
+// rule): 1 wave ran the near pass in 5.5 against 3.3 ms and the fold in
+// 22.3 against 17.9 ms, 3 waves the fold in 22.6 ms.
 #ifndef MHS_FOLD_NEAR_WAVES
 #define MHS_FOLD_NEAR_WAVES 2  // the fold's near-point kernel
 #endif
