@@ -67,7 +67,8 @@
 // and 3 or 4 (spills), measured at n = 10^4: fold near points 47 -> 37 ms,
 // near pass 18 -> 12.7 ms; measured again on the current kernels (mixed
 // rule): 1 wave ran the near pass in 5.5 against 3.3 ms and the fold in
-// 22.3 against 17.9 ms, 3 waves the fold in 22.6 ms.
+// 22.3 against 17.9 ms; 3 waves ran the fold in 22.6 against 21.1 ms (a
+// build before the fold's own switch).
 #ifndef MHS_FOLD_NEAR_WAVES
 #define MHS_FOLD_NEAR_WAVES 2  // the fold's near-point kernel
 #endif
