@@ -72,6 +72,25 @@ RULE_CANDIDATES = (
 #: of max |mixed - FP64| / max |FP64| in the row (the mixed mode's stated
 #: bound is 1e-6 per row).
 RULE_ERROR_BOUND = 1e-7
+#: Candidates within the bound whose sample time is within this fraction of
+#: the fastest count as tied, and the most accurate of them wins: the sample
+#: separates near-equal rules by less than a full step's run-to-run spread
+#: (a rule with a 3 L first band sampled 0.4% faster than the 5 L one with
+#: 5x its error, 9.6e-8 against 1.8e-8, and ran the step no faster).
+RULE_TIE = 0.01
+
+
+def pick_rule(table, bound: float = RULE_ERROR_BOUND, tie: float = RULE_TIE, fallback=None):
+    """The tuner's choice from ``table`` = {candidate: (error, ms)}: of the
+    candidates with error <= bound, those within ``tie`` of the fastest
+    time, the one with the least error (ties in error: the faster, then the
+    earlier); ``fallback`` when none is within the bound."""
+    ok = [(c, e, t) for c, (e, t) in table.items() if e <= bound]
+    if not ok:
+        return fallback
+    fastest = min(t for _, _, t in ok)
+    tied = [(e, t, i, c) for i, (c, e, t) in enumerate(ok) if t <= fastest * (1.0 + tie)]
+    return min(tied, key=lambda x: x[:3])[3]
 
 
 def _rule_name(rule) -> str:
@@ -492,9 +511,10 @@ class Prepared:
         candidate of RULE_CANDIDATES is run in the mixed mode on the same
         rows, its error (the largest over the rows of max|mixed - FP64| /
         max|FP64| in the row) measured and its passes timed (HIP events,
-        the fastest of AUTOTUNE_REPEATS launches after one warm launch); the
-        fastest candidate within
-        RULE_ERROR_BOUND wins (none within: the last, RULE_SAFE).  Cached per
+        the fastest of AUTOTUNE_REPEATS launches after one warm launch); of
+        the candidates within RULE_ERROR_BOUND, the most accurate of those
+        within RULE_TIE of the fastest wins (:func:`pick_rule`; none within
+        the bound: the last, RULE_SAFE).  Cached per
         rule_key() on the device with every candidate's error and time.
         Only the on-fault interaction (``job.self_rows``) is tuned: other
         forms take the library's default, RULE_SAFE, whose error is bounded
@@ -550,8 +570,7 @@ class Prepared:
                 err = max(float(((self.final[lo:hi] - r).abs().amax(dim=red) / sc).max())
                           for (lo, hi), r, sc in zip(blocks, ref, scale))
                 table[cand] = (err, best_ms)
-        ok = [c for c, (err, _) in table.items() if err <= RULE_ERROR_BOUND]
-        best = min(ok, key=lambda c: table[c][1]) if ok else RULE_CANDIDATES[-1]
+        best = pick_rule(table, fallback=RULE_CANDIDATES[-1])
         dev.rules[key] = best
         dev.rule_table[key] = {_rule_name(c): {"relmax": e, "ms": t} for c, (e, t) in table.items()}
         self.rule = rule_struct(best)

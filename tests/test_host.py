@@ -194,3 +194,30 @@ def test_fold_work_bound_covers_the_band():
     h = flops.histogram(obs, tris, eps)
     assert h["fold_pairs"] > 0
     assert runtime.fold_pairs_bound(obs, recs, strad) == h["fold_pairs"] + 64
+
+
+def test_rule_tuner_prefers_the_most_accurate_of_near_equal_rules():
+    """The mixed rule tuner's choice (runtime.pick_rule): candidates beyond
+    the error bound never win; among those within it, sample times within
+    RULE_TIE of the fastest are tied and the smallest error wins; a clearly
+    faster candidate wins over a more accurate one; nothing within the bound
+    gives the fallback.  The table is a measured one (interaction n=10k,
+    sample errors and ms of four rules)."""
+    from mhs_amd import runtime as R
+    measured = {
+        "5 L first band": (1.82e-8, 7.651),
+        "3 L first band": (9.64e-8, 7.620),
+        "3.5 L first band": (3.25e-8, 7.652),
+        "3 L switch": (7.22e-9, 7.878),
+        "1 L fold switch": (4.6e-7, 7.10),
+    }
+    assert R.pick_rule(measured, bound=1e-7, tie=0.01) == "5 L first band"
+    # a wider tie reaches the 3 L switch (3.4% slower, most accurate)
+    assert R.pick_rule(measured, bound=1e-7, tie=0.05) == "3 L switch"
+    # no tie: the fastest within the bound
+    assert R.pick_rule(measured, bound=1e-7, tie=0.0) == "3 L first band"
+    # the bound excludes the fastest of all
+    assert R.pick_rule(measured, bound=1e-6, tie=0.0) == "1 L fold switch"
+    assert R.pick_rule({"a": (2e-7, 1.0)}, bound=1e-7, fallback="safe") == "safe"
+    # equal errors: the faster, then the earlier
+    assert R.pick_rule({"a": (1e-8, 1.005), "b": (1e-8, 1.0), "c": (1e-8, 1.0)}, tie=0.01) == "b"
