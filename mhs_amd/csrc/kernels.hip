@@ -72,6 +72,14 @@
 #ifndef MHS_FOLD_NEAR_WAVES
 #define MHS_FOLD_NEAR_WAVES 2  // the fold's near-point kernel
 #endif
+// The near pass's blocks are one wave: a row tile holds few near pairs under
+// a near switch of a few L (about 50 per receiver row at n = 10^4 and 2 L),
+// so a four-wave block kept one wave of closed-form work and three idle
+// waves resident; one-wave blocks put four times the rows in flight.  The
+// queue form keeps four-wave blocks.
+#ifndef MHS_NEAR_THREADS
+#define MHS_NEAR_THREADS 64
+#endif
 #ifndef MHS_NEAR_WAVES
 #define MHS_NEAR_WAVES 2       // the near pass
 #endif
@@ -225,7 +233,8 @@ __device__ __forceinline__ void drain_queue(const QueueArgs& q, Body body) {
 #define MHS_ROW_TILE 2048
 
 // skip(first column of a step) may declare a whole step of 256 columns
-// class 0 without classifying them (the same answer in every lane).
+// class 0 without classifying them (the same answer in every thread of the
+// block: it may depend on the block's row and the step only).
 struct NoSkip {
   __device__ __forceinline__ bool operator()(int64_t) const { return false; }
 };
@@ -237,8 +246,14 @@ __device__ __forceinline__ void list_row_tile(int64_t n, int64_t c0, Cls cls, in
   const int tid = threadIdx.x, wave = tid >> 6;
   n1 = n2 = 0;
   for (int step = 0; step < MHS_ROW_TILE; step += 256) {
+    // a skipped step lists nothing: every thread of the block takes this
+    // branch together (skip depends on the step alone), so no barrier runs
+    if (skip(c0 + step)) {
+      if (c0 + step + 256 >= n) break;
+      continue;
+    }
     const int64_t col = c0 + step + tid;
-    const int c = (col < n && !skip(c0 + step)) ? cls(col) : 0;
+    const int c = col < n ? cls(col) : 0;
     const unsigned long long m1 = __ballot(c == 1), m2 = __ballot(c == 2);
     if ((tid & 63) == 0) {
       cnt_s[0][wave] = __popcll(m1);
@@ -264,6 +279,33 @@ __device__ __forceinline__ void list_row_tile(int64_t n, int64_t c0, Cls cls, in
     n2 += s2;
     __syncthreads();          // cnt_s is rewritten by the next step
     if (c0 + step + 256 >= n) break;
+  }
+}
+
+// list_row_tile for a block of one wave (64 threads): the same two lists in
+// the same order, steps of 64 columns, no barrier inside the loop (the
+// caller's block barrier follows before the lists are read).
+template <typename Cls, typename Skip = NoSkip>
+__device__ __forceinline__ void list_row_tile_wave(int64_t n, int64_t c0, Cls cls, int* list, int& n1, int& n2,
+                                                   Skip skip = Skip()) {
+  const int tid = threadIdx.x;
+  n1 = n2 = 0;
+  for (int step = 0; step < MHS_ROW_TILE; step += 64) {
+    if (!skip(c0 + step)) {
+      const int64_t col = c0 + step + tid;
+      const int c = col < n ? cls(col) : 0;
+      const unsigned long long m1 = __ballot(c == 1), m2 = __ballot(c == 2);
+      if (c == 1)
+        list[n1 + __builtin_amdgcn_mbcnt_hi((unsigned)(m1 >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m1, 0u))] =
+            step + tid;
+      if (c == 2)
+        list[MHS_ROW_TILE + n2 +
+             __builtin_amdgcn_mbcnt_hi((unsigned)(m2 >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m2, 0u))] =
+            step + tid;
+      n1 += __popcll(m1);
+      n2 += __popcll(m2);
+    }
+    if (c0 + step + 64 >= n) break;
   }
 }
 
@@ -1123,8 +1165,9 @@ __device__ __attribute__((noinline)) void near_item_call(const Params& pr, int64
 // columns of a source from one set of closed-form raw moments held in scratch
 // was faster at order 2 while the image edge integrals ran as a memory-bound
 // loop, and slower once they were unrolled: 84 against 66 ms at n = 1560.)
-template <int P, int NM>
+template <int P, int NM, int NT>
 __device__ __forceinline__ void body_near(const Params& pr, int64_t bx_, int64_t by_) {
+  static_assert(NT == 64 || NT == 256, "a near block is one wave or four");
   constexpr int K = (P + 1) * (P + 2) / 2;
   __shared__ int list_s[2 * MHS_ROW_TILE];
   const int64_t o = pr.obs_lo + by_;
@@ -1142,28 +1185,34 @@ __device__ __forceinline__ void body_near(const Params& pr, int64_t bx_, int64_t
     const double dz = fmax(fmax(t[2] - ob[2], ob[2] - t[5]), 0.0);
     return sqrt(dx * dx + dy * dy + dz * dz) > pr.near_r * t[6] * (1.0 + 1e-9);
   };
-  list_row_tile(pr.n_cols, c0, [&](int64_t col) {
+  auto cls = [&](int64_t col) {
     const int parts = near_parts(pr, pr.src + (col / K) * MHS_SRC_STRIDE, ob);
     return (parts & 2) ? 1 : parts == 1 ? 2 : 0;
-  }, list_s, n_img, n_dir, skip);
-  for (int i = threadIdx.x; i < n_img + n_dir; i += 256) {
+  };
+  if constexpr (NT == 64) {
+    list_row_tile_wave(pr.n_cols, c0, cls, list_s, n_img, n_dir, skip);
+    __syncthreads();
+  } else {
+    list_row_tile(pr.n_cols, c0, cls, list_s, n_img, n_dir, skip);
+  }
+  for (int i = threadIdx.x; i < n_img + n_dir; i += NT) {
     __asm__ volatile("" ::: "memory");   // no loads hoisted out of the pair loop
     const int c = i < n_img ? list_s[i] : list_s[MHS_ROW_TILE + i - n_img];
     near_item_call<P, NM>(pr, o, c0 + c);
   }
 }
 template <int P, int NM>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MHS_NEAR_WAVES)))
+__global__ void __launch_bounds__(MHS_NEAR_THREADS) __attribute__((amdgpu_waves_per_eu(MHS_NEAR_WAVES)))
 k_near(Params pr) {
   int64_t bx, by;
   rows_first(bx, by);
-  body_near<P, NM>(pr, bx, by);
+  body_near<P, NM, MHS_NEAR_THREADS>(pr, bx, by);
 }
 // the same pass over the tiles of a batch's queue (see drain_queue)
 template <int P, int NM>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MHS_NEAR_WAVES)))
 kq_near(QueueArgs q) {
-  drain_queue(q, [&](const Params& pr, int64_t bx, int64_t by) { body_near<P, NM>(pr, bx, by); });
+  drain_queue(q, [&](const Params& pr, int64_t bx, int64_t by) { body_near<P, NM, 256>(pr, bx, by); });
 }
 
 __global__ void k_zero_i32(int* p, int n) {
@@ -1832,8 +1881,8 @@ static int launch_item(const PlanItem& it, hipStream_t st) {
     case V_FARM_2: hipLaunchKernelGGL((k_far_mixed<P, 2>), g, b, 0, st, it.pr); break;
     case V_QM_S: hipLaunchKernelGGL((k_qm<P, true>), g, b, 0, st, it.pr); break;
     case V_QM_D: hipLaunchKernelGGL((k_qm<P, false>), g, b, 0, st, it.pr); break;
-    case V_NEAR: hipLaunchKernelGGL((k_near<P, 2>), g, b, 0, st, it.pr); break;
-    default: hipLaunchKernelGGL((k_near<P, 3>), g, b, 0, st, it.pr); break;
+    case V_NEAR: hipLaunchKernelGGL((k_near<P, 2>), g, dim3(MHS_NEAR_THREADS), 0, st, it.pr); break;
+    default: hipLaunchKernelGGL((k_near<P, 3>), g, dim3(MHS_NEAR_THREADS), 0, st, it.pr); break;
   }
   return check(hipGetLastError(), "mhs_assemble: pass launch");
 }
