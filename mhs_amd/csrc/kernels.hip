@@ -72,13 +72,15 @@
 #ifndef MHS_FOLD_NEAR_WAVES
 #define MHS_FOLD_NEAR_WAVES 2  // the fold's near-point kernel
 #endif
-// The near pass's blocks are one wave: a row tile holds few near pairs under
-// a near switch of a few L (about 50 per receiver row at n = 10^4 and 2 L),
-// so a four-wave block kept one wave of closed-form work and three idle
-// waves resident; one-wave blocks put four times the rows in flight.  The
-// queue form keeps four-wave blocks.
+// Threads per near-pass block: 256 (four waves), or 64 (one wave).  At
+// n = 10^4 under the mixed rule's 2 L switch a row tile holds a few dozen
+// near pairs, and one-wave blocks ran the near pass in 2.41 against 3.30 ms
+// (FP64 rule 7.33 against 8.45 ms), bitwise the same; but the served stream
+// of 512 requests lost with them (mixed p99 3.6 against 2.65 s, mean 0.26
+// against 0.20 s, two runs each, same frozen stream), so four waves stay
+// the default.  The queue form always takes four-wave blocks.
 #ifndef MHS_NEAR_THREADS
-#define MHS_NEAR_THREADS 64
+#define MHS_NEAR_THREADS 256
 #endif
 #ifndef MHS_NEAR_WAVES
 #define MHS_NEAR_WAVES 2       // the near pass
